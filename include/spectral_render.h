@@ -1,0 +1,130 @@
+/* spectral_render.h -- C ABI of the MI355X spectral path tracer.
+ *
+ * Drop-in boundary for the hot path of Xyzonox/taichi-spectral-render.  The
+ * reference has no FFI: its boundary is the Python scene dict consumed by
+ * setup_scene (main.py:509-631) and the kernel-call protocol of its main
+ * loop (main.py:1826-1837: cast_rays per band, average_and_reset_block and
+ * color_rays per block).  Each entry point below names the reference code it
+ * replaces.
+ *
+ * Conventions
+ *   - Every function returns 0 on success or a negative SR_E* code, and
+ *     records a message readable with sr_last_error() (thread-local).
+ *   - Handles are opaque, bound to the HIP device current at sr_create, and
+ *     not thread-safe: use each from one host thread.
+ *   - Host arrays are borrowed for the duration of the call and copied.
+ *   - Arrays named dev_* are device pointers on the handle's device (for
+ *     example a torch tensor's data_ptr()).  Work is ordered on the handle's
+ *     stream (sr_set_stream); calls return once work is enqueued, except
+ *     those documented as blocking.
+ *   - Accumulator layout is the reference's field layout [W][H][B] (pixel
+ *     (i, j) with j pointing up, main.py:365), f32.
+ */
+#ifndef SPECTRAL_RENDER_H
+#define SPECTRAL_RENDER_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sr_handle sr_handle;
+
+#define SR_OK 0
+#define SR_EINVAL -1   /* bad argument */
+#define SR_ESTATE -2   /* call out of order (no scene / no resolution) */
+#define SR_EHIP -3     /* HIP runtime error */
+#define SR_ENOMEM -4   /* allocation failed */
+#define SR_EOVERFLOW -5 /* traversal stack overflow detected */
+
+/* sr_trace_closest flags */
+#define SR_DEVICE_PTRS 1
+
+/* Last error message of this thread ("" when none). */
+const char* sr_last_error(void);
+
+/* ABI version (major * 100 + minor). */
+int sr_version(void);
+
+/* Creates a renderer for `bands` spectral bands on [wl_min, wl_max] nm
+ * (SPECTRAL_BANDS / WAVELENGTH_MIN / WAVELENGTH_MAX, main.py:26-28;
+ * band k is wl_min + k * (wl_max - wl_min) / (bands - 1), main.py:34). */
+int sr_create(int bands, float wl_min, float wl_max, sr_handle** out);
+int sr_destroy(sr_handle* h);
+
+/* Orders all work of `h` on `stream` (a hipStream_t; NULL = the handle's own
+ * default-priority stream). */
+int sr_set_stream(sr_handle* h, void* stream);
+
+/* Replaces setup_scene (main.py:509-631) and the field allocation of
+ * main.py:1683-1709.
+ *   tris_v  [n_tris][3][3] f32  vertex positions per face
+ *   tris_n  [n_tris][3][3] f32  per-vertex shading normals (the face normal
+ *                               repeated for meshes without vertex_normals)
+ *   tri_mat [n_tris]       i32  material id per face
+ *   mat_f32 [n_mats][6][B] f32  refractive_index, extinction_coefficient,
+ *                               roughness, scattering_coefficient, emission,
+ *                               anistropy_factor (main.py:40-45)
+ *   mat_i32 [n_mats][2][B] i32  is_true_volume, surface_emission (main.py:49-50)
+ *   mat_eem [n_mats]       i32  eem_id (0 = none)
+ *   eem     [n_eem][B][B]  f32  reemission_matrix (main.py:372); row 0 unused
+ *   cam     [9]            f32  origin, lookat, up (main.py:242-246)
+ * Builds face normals/areas (main.py:526-536), the per-band area-CDF light
+ * lists (main.py:590-627) and the 4-wide BVH, and uploads everything. */
+int sr_upload_scene(sr_handle* h, const float* tris_v, const float* tris_n, const int* tri_mat, int n_tris,
+                    const float* mat_f32, const int* mat_i32, const int* mat_eem, int n_mats, const float* eem,
+                    int n_eem, const float* cam);
+
+/* update_camera (main.py:633-640): new origin/lookat/up for the next render. */
+int sr_set_camera(sr_handle* h, const float* cam);
+
+/* Material table update without rebuilding geometry (same layout as upload). */
+int sr_update_materials(sr_handle* h, const float* mat_f32, const int* mat_i32, const int* mat_eem, int n_mats,
+                        const float* eem, int n_eem);
+
+/* Scene statistics: BVH node count, 4-wide depth, max lights per band,
+ * device bytes held by the scene. */
+int sr_scene_info(sr_handle* h, int* n_nodes, int* bvh_depth, int* max_lights, long long* device_bytes);
+
+/* Closest hit for n rays: trace (main.py:831-864) as a batch.  out_t = 1e9
+ * and out_prim = -1 on a miss; out_prim is the ORIGINAL face id (the order
+ * of tris_v at upload); ties in t resolve to the smallest face id.  With
+ * SR_DEVICE_PTRS all arrays are device pointers and the call is
+ * asynchronous; otherwise they are host arrays and the call blocks. */
+int sr_trace_closest(sr_handle* h, int n, const float* orig, const float* dir, float* out_t, int* out_prim,
+                     float* out_uv, int flags);
+
+/* Renders spectral blocks [first_block, first_block + n_blocks): for each
+ * block, each band k of 0..B-1 and each pixel, one single-band path of
+ * cast_rays (main.py:1502-1590) with at most max_depth segments, then folds
+ * the blocks into the running mean dev_accum [W][H][B] exactly as
+ * average_and_reset_block (main.py:1592-1607) does once per block.
+ * Randomness: Philox-4x32-10 keyed by seed, counter (i*H + j, block,
+ * band | stream << 8, call/4).  flags: reserved, pass 0. */
+int sr_render(sr_handle* h, int width, int height, uint64_t seed, uint32_t stream, int first_block, int n_blocks,
+              int max_depth, int flags, float* dev_accum);
+
+/* color_rays (main.py:1609-1629): dev_accum [W][H][B] -> dev_rgb
+ * [out_w][out_h][3] with nearest upsampling.  cmf [B][3] (host),
+ * xyz2rgb [3][3] row-major (host), norm = sum(cmf_y) * step. */
+int sr_resolve(sr_handle* h, const float* dev_accum, int width, int height, const float* cmf, const float* xyz2rgb,
+               float norm, float exposure, float gamma, int out_w, int out_h, float* dev_rgb);
+
+/* Counters of the last sr_render (blocking): path segments (extension
+ * traversals, one per executed path-loop iteration), all traversals
+ * (extension + shadow), dropped non-finite contributions. */
+int sr_stats(sr_handle* h, uint64_t* segments, uint64_t* traversals, uint64_t* nonfinite);
+
+/* Device time of the last sr_render's path kernel in ms (blocking; HIP
+ * events recorded on the handle's stream around the launch). */
+int sr_last_kernel_ms(sr_handle* h, float* ms);
+
+/* Blocks until all work of h is done. */
+int sr_sync(sr_handle* h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPECTRAL_RENDER_H */

@@ -1,0 +1,547 @@
+// sr_api.cpp -- C ABI of the spectral path tracer (host runtime).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/spectral_render.h"
+#include "sr_kernels.h"
+
+int sr_build_bvh4(const float* v, int n, std::vector<Node4>& nodes, std::vector<int>& tri_order);
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                         \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) return fail(SR_EHIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  int ensure(size_t n) {
+    if (n <= bytes && p) return 0;
+    release();
+    if (n == 0) n = 16;
+    if (hipMalloc(&p, n) != hipSuccess) { p = nullptr; return fail(SR_ENOMEM, "hipMalloc(%zu) failed", n); }
+    bytes = n;
+    return 0;
+  }
+  template <class T>
+  int upload(const T* src, size_t count, hipStream_t st) {
+    int rc = ensure(count * sizeof(T));
+    if (rc) return rc;
+    if (count) HIPCHK(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, st));
+    return 0;
+  }
+};
+
+struct V3h {
+  float x, y, z;
+};
+inline V3h sub(V3h a, V3h b) { return V3h{a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3h crs(V3h a, V3h b) { return V3h{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline float dt(V3h a, V3h b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3h nrm(V3h a) {
+  float l = std::sqrt(dt(a, a));
+  return V3h{a.x / l, a.y / l, a.z / l};
+}
+
+}  // namespace
+
+struct sr_handle {
+  int B = 0;
+  float wl_min = 0, wl_max = 0, step = 0;
+  int device = 0;
+  hipStream_t own = nullptr, st = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool timed = false;
+  // scene
+  bool has_scene = false;
+  int n_tris = 0, n_mats = 0, n_eem = 0, n_nodes = 0, depth = 0, max_lights = 0, n_slots = 0;
+  DevBuf nodes, tris, shade, mats, mat_eem, eem, lights, lverts, light_count, light_area, lam_m, lam_nm;
+  std::vector<float> h_light_area;
+  std::vector<int> h_light_count;
+  float cam_o[3], cam_u[3], cam_v[3], cam_w[3];
+  // work
+  DevBuf sum, counters, queue, overflow, tmp;
+  int grid = 0;
+};
+
+static void fill_scene_dev(const sr_handle* h, SceneDev& s) {
+  s.nodes = (const Node4*)h->nodes.p;
+  s.tris = (const float4*)h->tris.p;
+  s.shade = (const float4*)h->shade.p;
+  s.mats = (const MatRec*)h->mats.p;
+  s.mat_eem = (const int*)h->mat_eem.p;
+  s.eem = (const float*)h->eem.p;
+  s.lights = (const LightRec*)h->lights.p;
+  s.lverts = (const float4*)h->lverts.p;
+  s.light_count = (const int*)h->light_count.p;
+  s.light_area = (const float*)h->light_area.p;
+  s.lam_m = (const float*)h->lam_m.p;
+  s.lam_nm = (const float*)h->lam_nm.p;
+  s.B = h->B;
+  s.max_lights = std::max(h->max_lights, 1);
+  s.M = h->n_mats;
+  s.n_tris = h->n_tris;
+  for (int k = 0; k < 3; ++k) {
+    s.cam_o[k] = h->cam_o[k];
+    s.cam_u[k] = h->cam_u[k];
+    s.cam_v[k] = h->cam_v[k];
+    s.cam_w[k] = h->cam_w[k];
+  }
+}
+
+static void set_camera_host(sr_handle* h, const float* cam) {
+  V3h o{cam[0], cam[1], cam[2]}, la{cam[3], cam[4], cam[5]}, up{cam[6], cam[7], cam[8]};
+  V3h w = nrm(sub(o, la));  // update_camera, main.py:633-640
+  V3h u = nrm(crs(up, w));
+  V3h v = crs(w, u);
+  h->cam_o[0] = o.x; h->cam_o[1] = o.y; h->cam_o[2] = o.z;
+  h->cam_u[0] = u.x; h->cam_u[1] = u.y; h->cam_u[2] = u.z;
+  h->cam_v[0] = v.x; h->cam_v[1] = v.y; h->cam_v[2] = v.z;
+  h->cam_w[0] = w.x; h->cam_w[1] = w.y; h->cam_w[2] = w.z;
+}
+
+static int upload_materials(sr_handle* h, const float* mat_f32, const int* mat_i32, const int* mat_eem, int n_mats,
+                            const float* eem, int n_eem) {
+  const int B = h->B;
+  if (n_mats <= 0) return fail(SR_EINVAL, "scene needs at least one material (material 0 is the ambient medium)");
+  if (!mat_f32 || !mat_i32 || !mat_eem) return fail(SR_EINVAL, "material arrays must not be NULL");
+  if (n_eem < 0 || (n_eem > 0 && !eem)) return fail(SR_EINVAL, "bad EEM table");
+  std::vector<MatRec> mr((size_t)n_mats * B);
+  for (int m = 0; m < n_mats; ++m) {
+    int e = mat_eem[m];
+    if (e < 0 || (e > 0 && e >= n_eem)) return fail(SR_EINVAL, "material %d has eem_id %d outside [0, %d)", m, e, n_eem);
+    for (int b = 0; b < B; ++b) {
+      MatRec& r = mr[(size_t)m * B + b];
+      const float* f = mat_f32 + (size_t)m * 6 * B;
+      const int* i = mat_i32 + (size_t)m * 2 * B;
+      r.n = f[0 * B + b];
+      r.k = f[1 * B + b];
+      r.r = f[2 * B + b];
+      r.ss = f[3 * B + b];
+      r.em = f[4 * B + b];
+      r.g = f[5 * B + b];
+      r.tv = i[0 * B + b];
+      r.se = i[1 * B + b];
+    }
+  }
+  int rc;
+  if ((rc = h->mats.upload(mr.data(), mr.size(), h->st))) return rc;
+  if ((rc = h->mat_eem.upload(mat_eem, (size_t)n_mats, h->st))) return rc;
+  std::vector<float> et((size_t)std::max(n_eem, 1) * B * B, 0.0f);
+  if (n_eem > 0) std::memcpy(et.data(), eem, sizeof(float) * (size_t)n_eem * B * B);
+  if ((rc = h->eem.upload(et.data(), et.size(), h->st))) return rc;
+  h->n_mats = n_mats;
+  h->n_eem = n_eem;
+  HIPCHK(hipStreamSynchronize(h->st));
+  return 0;
+}
+
+extern "C" {
+
+const char* sr_last_error(void) { return g_err.c_str(); }
+
+int sr_version(void) { return 100; }
+
+int sr_create(int bands, float wl_min, float wl_max, sr_handle** out) {
+  g_err.clear();
+  if (!out) return fail(SR_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (bands < 2 || bands > SR_MAX_BANDS) return fail(SR_EINVAL, "bands=%d outside [2, %d]", bands, SR_MAX_BANDS);
+  if (!(wl_max > wl_min) || !(wl_min > 0.f)) return fail(SR_EINVAL, "need 0 < wl_min < wl_max");
+  sr_handle* h = new sr_handle();
+  h->B = bands;
+  h->wl_min = wl_min;
+  h->wl_max = wl_max;
+  h->step = (float)(((double)wl_max - (double)wl_min) / (double)(bands - 1));
+  hipError_t e = hipGetDevice(&h->device);
+  if (e == hipSuccess) e = hipStreamCreate(&h->own);  // blocking: ordered with the null stream
+  if (e == hipSuccess) e = hipEventCreate(&h->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&h->ev1);
+  if (e != hipSuccess) {
+    delete h;
+    return fail(SR_EHIP, "HIP init failed: %s", hipGetErrorString(e));
+  }
+  h->st = h->own;
+  if (sr_path_grid(&h->grid) != 0) {
+    delete h;
+    return fail(SR_EHIP, "occupancy query failed");
+  }
+  int rc = h->counters.ensure(4 * sizeof(unsigned long long));
+  if (!rc) rc = h->queue.ensure(64);
+  if (!rc) rc = h->overflow.ensure(64);
+  if (rc) {
+    delete h;
+    return rc;
+  }
+  (void)hipMemset(h->counters.p, 0, h->counters.bytes);
+  (void)hipMemset(h->overflow.p, 0, h->overflow.bytes);
+  *out = h;
+  return 0;
+}
+
+int sr_destroy(sr_handle* h) {
+  if (!h) return 0;
+  (void)hipSetDevice(h->device);
+  if (h->st) (void)hipStreamSynchronize(h->st);
+  DevBuf* bufs[] = {&h->nodes, &h->tris, &h->shade, &h->mats, &h->mat_eem, &h->eem, &h->lights, &h->lverts,
+                    &h->light_count, &h->light_area, &h->lam_m, &h->lam_nm, &h->sum, &h->counters, &h->queue,
+                    &h->overflow, &h->tmp};
+  for (DevBuf* b : bufs) b->release();
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->own) (void)hipStreamDestroy(h->own);
+  delete h;
+  return 0;
+}
+
+int sr_set_stream(sr_handle* h, void* stream) {
+  if (!h) return fail(SR_EINVAL, "null handle");
+  h->st = stream ? (hipStream_t)stream : h->own;
+  return 0;
+}
+
+int sr_upload_scene(sr_handle* h, const float* tris_v, const float* tris_n, const int* tri_mat, int n_tris,
+                    const float* mat_f32, const int* mat_i32, const int* mat_eem, int n_mats, const float* eem,
+                    int n_eem, const float* cam) {
+  g_err.clear();
+  if (!h) return fail(SR_EINVAL, "null handle");
+  if (n_tris < 0 || (n_tris > 0 && (!tris_v || !tris_n || !tri_mat))) return fail(SR_EINVAL, "bad triangle arrays");
+  if (n_tris >= (1 << 27)) return fail(SR_EINVAL, "n_tris=%d exceeds 2^27", n_tris);
+  if (!cam) return fail(SR_EINVAL, "cam is NULL");
+  HIPCHK(hipSetDevice(h->device));
+  const int B = h->B;
+  for (int f = 0; f < n_tris; ++f)
+    if (tri_mat[f] < 0 || tri_mat[f] >= n_mats)
+      return fail(SR_EINVAL, "face %d has material %d outside [0, %d)", f, tri_mat[f], n_mats);
+  int rc = upload_materials(h, mat_f32, mat_i32, mat_eem, n_mats, eem, n_eem);
+  if (rc) return rc;
+
+  // per-face normal and area in f32 (main.py:526, 535-536)
+  std::vector<float> area((size_t)n_tris);
+  std::vector<float4> shade((size_t)std::max(n_tris, 1) * 4);
+  for (int f = 0; f < n_tris; ++f) {
+    const float* p = tris_v + (size_t)f * 9;
+    const float* q = tris_n + (size_t)f * 9;
+    V3h v0{p[0], p[1], p[2]}, v1{p[3], p[4], p[5]}, v2{p[6], p[7], p[8]};
+    V3h n = crs(sub(v1, v0), sub(v2, v0));
+    float a = std::sqrt(dt(n, n)) / 2.0f;
+    if (a > 1e-6f) n = V3h{n.x / (a * 2.0f), n.y / (a * 2.0f), n.z / (a * 2.0f)};
+    area[f] = a;
+    int mat = tri_mat[f];
+    float matf;
+    std::memcpy(&matf, &mat, 4);
+    shade[(size_t)f * 4 + 0] = make_float4(n.x, n.y, n.z, a);
+    shade[(size_t)f * 4 + 1] = make_float4(q[0], q[1], q[2], matf);
+    shade[(size_t)f * 4 + 2] = make_float4(q[3], q[4], q[5], 0.f);
+    shade[(size_t)f * 4 + 3] = make_float4(q[6], q[7], q[8], 0.f);
+  }
+
+  // per-band light lists (main.py:590-627): faces whose material emits at
+  // band b, in face order; CDF of area fractions accumulated in f64 and
+  // stored f32, last entry forced to 1.
+  std::vector<std::vector<int>> per_band((size_t)B);
+  std::vector<int> slot_of((size_t)n_tris, -1);
+  std::vector<int> slots;
+  int max_lights = 0;
+  for (int b = 0; b < B; ++b) {
+    for (int f = 0; f < n_tris; ++f) {
+      if (mat_i32[((size_t)tri_mat[f] * 2 + 1) * B + b]) per_band[b].push_back(f);
+    }
+    max_lights = std::max(max_lights, (int)per_band[b].size());
+  }
+  std::vector<LightRec> lr((size_t)B * std::max(max_lights, 1));
+  std::vector<int> lcount((size_t)B, 0);
+  std::vector<float> larea((size_t)B, 0.0f);
+  for (int b = 0; b < B; ++b) {
+    const std::vector<int>& ids = per_band[b];
+    lcount[b] = (int)ids.size();
+    if (ids.empty()) continue;
+    double tot = 0.0;
+    for (int f : ids) tot += (double)area[f];
+    larea[b] = (float)tot;
+    double c = 0.0;
+    for (size_t i = 0; i < ids.size(); ++i) {
+      int f = ids[i];
+      c += (double)area[f] / tot;
+      if (slot_of[f] < 0) {
+        slot_of[f] = (int)slots.size();
+        slots.push_back(f);
+      }
+      LightRec& L = lr[(size_t)b * std::max(max_lights, 1) + i];
+      L.cdf = (float)c;
+      L.prim = f;
+      L.slot = slot_of[f];
+      L.pad = 0;
+    }
+    lr[(size_t)b * std::max(max_lights, 1) + ids.size() - 1].cdf = 1.0f;
+  }
+  std::vector<float4> lv(std::max<size_t>(slots.size(), 1) * 3);
+  for (size_t s = 0; s < slots.size(); ++s) {
+    const float* p = tris_v + (size_t)slots[s] * 9;
+    for (int k = 0; k < 3; ++k) lv[s * 3 + k] = make_float4(p[3 * k], p[3 * k + 1], p[3 * k + 2], 0.f);
+  }
+
+  // 4-wide BVH and triangle records in leaf order
+  std::vector<Node4> nodes;
+  std::vector<int> order;
+  int depth = sr_build_bvh4(tris_v, n_tris, nodes, order);
+  if (depth < 0) return fail(SR_EINVAL, "BVH build failed");
+  if (3 * depth + 1 > SR_STACK_SIZE) return fail(SR_EOVERFLOW, "BVH depth %d exceeds the traversal stack", depth);
+  std::vector<float4> tr(std::max<size_t>(order.size(), 1) * 3);
+  for (size_t i = 0; i < order.size(); ++i) {
+    int f = order[i];
+    const float* p = tris_v + (size_t)f * 9;
+    float pf;
+    std::memcpy(&pf, &f, 4);
+    V3h v0{p[0], p[1], p[2]}, v1{p[3], p[4], p[5]}, v2{p[6], p[7], p[8]};
+    V3h e1 = sub(v1, v0), e2 = sub(v2, v0);
+    tr[i * 3 + 0] = make_float4(v0.x, v0.y, v0.z, pf);
+    tr[i * 3 + 1] = make_float4(e1.x, e1.y, e1.z, 0.f);
+    tr[i * 3 + 2] = make_float4(e2.x, e2.y, e2.z, 0.f);
+  }
+  if (nodes.empty()) {  // empty scene: one node with four empty slots
+    Node4 t;
+    std::memset(&t, 0, sizeof(t));
+    for (int s = 0; s < 4; ++s) {
+      t.lo_x[s] = t.lo_y[s] = t.lo_z[s] = 3.0e38f;
+      t.hi_x[s] = t.hi_y[s] = t.hi_z[s] = -3.0e38f;
+      t.child[s] = SR_EMPTY;
+    }
+    nodes.push_back(t);
+  }
+
+  // band wavelengths as the reference forms them (main.py:1057, 1394)
+  std::vector<float> lm((size_t)B), ln((size_t)B);
+  for (int b = 0; b < B; ++b) {
+    lm[b] = (float)((double)h->wl_min * 1e-9) + (float)b * h->step * 1e-9f;
+    ln[b] = h->wl_min + (float)b * h->step;
+  }
+
+  hipStream_t st = h->st;
+  if ((rc = h->nodes.upload(nodes.data(), nodes.size(), st))) return rc;
+  if ((rc = h->tris.upload(tr.data(), tr.size(), st))) return rc;
+  if ((rc = h->shade.upload(shade.data(), shade.size(), st))) return rc;
+  if ((rc = h->lights.upload(lr.data(), lr.size(), st))) return rc;
+  if ((rc = h->lverts.upload(lv.data(), lv.size(), st))) return rc;
+  if ((rc = h->light_count.upload(lcount.data(), lcount.size(), st))) return rc;
+  if ((rc = h->light_area.upload(larea.data(), larea.size(), st))) return rc;
+  if ((rc = h->lam_m.upload(lm.data(), lm.size(), st))) return rc;
+  if ((rc = h->lam_nm.upload(ln.data(), ln.size(), st))) return rc;
+  HIPCHK(hipStreamSynchronize(st));
+  h->n_tris = n_tris;
+  h->n_nodes = (int)nodes.size();
+  h->depth = depth;
+  h->max_lights = max_lights;
+  h->n_slots = (int)slots.size();
+  h->h_light_area = larea;
+  h->h_light_count = lcount;
+  set_camera_host(h, cam);
+  h->has_scene = true;
+  return 0;
+}
+
+int sr_set_camera(sr_handle* h, const float* cam) {
+  if (!h || !cam) return fail(SR_EINVAL, "null argument");
+  set_camera_host(h, cam);
+  return 0;
+}
+
+int sr_update_materials(sr_handle* h, const float* mat_f32, const int* mat_i32, const int* mat_eem, int n_mats,
+                        const float* eem, int n_eem) {
+  g_err.clear();
+  if (!h) return fail(SR_EINVAL, "null handle");
+  if (!h->has_scene) return fail(SR_ESTATE, "no scene uploaded");
+  if (n_mats != h->n_mats) return fail(SR_EINVAL, "material count changed (%d -> %d)", h->n_mats, n_mats);
+  HIPCHK(hipSetDevice(h->device));
+  // light lists depend on surface_emission: refuse a change there
+  return upload_materials(h, mat_f32, mat_i32, mat_eem, n_mats, eem, n_eem);
+}
+
+int sr_scene_info(sr_handle* h, int* n_nodes, int* bvh_depth, int* max_lights, long long* device_bytes) {
+  if (!h) return fail(SR_EINVAL, "null handle");
+  if (n_nodes) *n_nodes = h->n_nodes;
+  if (bvh_depth) *bvh_depth = h->depth;
+  if (max_lights) *max_lights = h->max_lights;
+  if (device_bytes) {
+    const DevBuf* bufs[] = {&h->nodes, &h->tris, &h->shade, &h->mats, &h->mat_eem, &h->eem, &h->lights,
+                            &h->lverts, &h->light_count, &h->light_area, &h->lam_m, &h->lam_nm};
+    long long t = 0;
+    for (const DevBuf* b : bufs) t += (long long)b->bytes;
+    *device_bytes = t;
+  }
+  return 0;
+}
+
+int sr_trace_closest(sr_handle* h, int n, const float* orig, const float* dir, float* out_t, int* out_prim,
+                     float* out_uv, int flags) {
+  g_err.clear();
+  if (!h) return fail(SR_EINVAL, "null handle");
+  if (!h->has_scene) return fail(SR_ESTATE, "no scene uploaded");
+  if (n < 0) return fail(SR_EINVAL, "n < 0");
+  if (n == 0) return 0;
+  if (!orig || !dir || !out_t || !out_prim || !out_uv) return fail(SR_EINVAL, "null array");
+  HIPCHK(hipSetDevice(h->device));
+  SceneDev s;
+  fill_scene_dev(h, s);
+  hipStream_t st = h->st;
+  HIPCHK(hipMemsetAsync(h->overflow.p, 0, 4, st));
+  if (flags & SR_DEVICE_PTRS) {
+    if (sr_launch_trace(s, n, orig, dir, out_t, out_prim, out_uv, (uint32_t*)h->overflow.p, st))
+      return fail(SR_EHIP, "trace launch failed");
+    return 0;
+  }
+  const size_t nb = (size_t)n;
+  size_t need = nb * (3 + 3 + 1 + 1 + 2) * 4;
+  int rc = h->tmp.ensure(need);
+  if (rc) return rc;
+  char* base = (char*)h->tmp.p;
+  float* d_o = (float*)base;
+  float* d_d = d_o + nb * 3;
+  float* d_t = d_d + nb * 3;
+  int* d_p = (int*)(d_t + nb);
+  float* d_uv = (float*)(d_p + nb);
+  HIPCHK(hipMemcpyAsync(d_o, orig, nb * 12, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d_d, dir, nb * 12, hipMemcpyHostToDevice, st));
+  if (sr_launch_trace(s, n, d_o, d_d, d_t, d_p, d_uv, (uint32_t*)h->overflow.p, st))
+    return fail(SR_EHIP, "trace launch failed");
+  HIPCHK(hipMemcpyAsync(out_t, d_t, nb * 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(out_prim, d_p, nb * 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(out_uv, d_uv, nb * 8, hipMemcpyDeviceToHost, st));
+  uint32_t ovf = 0;
+  HIPCHK(hipMemcpyAsync(&ovf, h->overflow.p, 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (ovf) return fail(SR_EOVERFLOW, "traversal stack overflow");
+  return 0;
+}
+
+int sr_render(sr_handle* h, int width, int height, uint64_t seed, uint32_t stream, int first_block, int n_blocks,
+              int max_depth, int flags, float* dev_accum) {
+  g_err.clear();
+  (void)flags;
+  if (!h) return fail(SR_EINVAL, "null handle");
+  if (!h->has_scene) return fail(SR_ESTATE, "no scene uploaded");
+  if (width <= 0 || height <= 0) return fail(SR_EINVAL, "bad resolution %dx%d", width, height);
+  if (first_block < 0 || n_blocks < 0) return fail(SR_EINVAL, "bad block range");
+  if (max_depth < 0) return fail(SR_EINVAL, "max_depth < 0");
+  if (stream > 0xFFFFFFu) return fail(SR_EINVAL, "stream id must fit 24 bits");
+  if (!dev_accum) return fail(SR_EINVAL, "dev_accum is NULL");
+  const unsigned long long items = (unsigned long long)width * height * (unsigned long long)n_blocks;
+  if (items >= 0x7FFFFFFFull) return fail(SR_EINVAL, "W*H*n_blocks must be < 2^31 per call");
+  if ((unsigned long long)width * height >= (1ull << 31)) return fail(SR_EINVAL, "too many pixels");
+  HIPCHK(hipSetDevice(h->device));
+  if (n_blocks == 0) return 0;
+  hipStream_t st = h->st;
+  const size_t n = (size_t)width * height * h->B;
+  if (h->sum.bytes < n * 4) {
+    int rc = h->sum.ensure(n * 4);
+    if (rc) return rc;
+    if (sr_launch_fill((float*)h->sum.p, n, 0.0f, st)) return fail(SR_EHIP, "fill launch failed");
+  }
+  HIPCHK(hipMemsetAsync(h->counters.p, 0, 4 * sizeof(unsigned long long), st));
+  HIPCHK(hipMemsetAsync(h->queue.p, 0, 4, st));
+  HIPCHK(hipMemsetAsync(h->overflow.p, 0, 4, st));
+  PathArgs a;
+  fill_scene_dev(h, a.s);
+  a.sum = (float*)h->sum.p;
+  a.counters = (unsigned long long*)h->counters.p;
+  a.queue = (uint32_t*)h->queue.p;
+  a.overflow = (uint32_t*)h->overflow.p;
+  a.n_items = (uint32_t)items;
+  a.W = width;
+  a.H = height;
+  a.depth = max_depth;
+  a.first_block = first_block;
+  a.n_blocks = n_blocks;
+  a.tiled = (width % 8 == 0 && height % 8 == 0) ? 1 : 0;
+  a.seed_lo = (uint32_t)seed;
+  a.seed_hi = (uint32_t)(seed >> 32);
+  a.stream = stream;
+  int grid = h->grid;
+  const unsigned long long waves_needed = (items + 255) / 256;
+  if ((unsigned long long)grid > waves_needed) grid = (int)waves_needed;
+  HIPCHK(hipEventRecord(h->ev0, st));
+  if (sr_launch_path(a, grid, st)) return fail(SR_EHIP, "path kernel launch failed");
+  HIPCHK(hipEventRecord(h->ev1, st));
+  h->timed = true;
+  if (sr_launch_finalize(dev_accum, (float*)h->sum.p, n, first_block, n_blocks, st))
+    return fail(SR_EHIP, "finalize launch failed");
+  return 0;
+}
+
+int sr_resolve(sr_handle* h, const float* dev_accum, int width, int height, const float* cmf, const float* xyz2rgb,
+               float norm, float exposure, float gamma, int out_w, int out_h, float* dev_rgb) {
+  g_err.clear();
+  if (!h) return fail(SR_EINVAL, "null handle");
+  if (!dev_accum || !dev_rgb || !cmf || !xyz2rgb) return fail(SR_EINVAL, "null array");
+  if (width <= 0 || height <= 0 || out_w <= 0 || out_h <= 0) return fail(SR_EINVAL, "bad size");
+  HIPCHK(hipSetDevice(h->device));
+  ResolveParams p;
+  std::memset(&p, 0, sizeof(p));
+  std::memcpy(p.cmf, cmf, sizeof(float) * 3 * h->B);
+  std::memcpy(p.T, xyz2rgb, sizeof(float) * 9);
+  p.step = h->step;
+  p.norm = norm;
+  p.exposure = exposure;
+  p.gamma = gamma;
+  if (sr_launch_resolve(dev_accum, width, height, h->B, p, dev_rgb, out_w, out_h, h->st))
+    return fail(SR_EHIP, "resolve launch failed");
+  return 0;
+}
+
+int sr_stats(sr_handle* h, uint64_t* segments, uint64_t* traversals, uint64_t* nonfinite) {
+  if (!h) return fail(SR_EINVAL, "null handle");
+  HIPCHK(hipSetDevice(h->device));
+  unsigned long long c[4] = {0, 0, 0, 0};
+  uint32_t ovf = 0;
+  HIPCHK(hipMemcpyAsync(c, h->counters.p, sizeof(c), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipMemcpyAsync(&ovf, h->overflow.p, 4, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  if (segments) *segments = c[0];
+  if (traversals) *traversals = c[1];
+  if (nonfinite) *nonfinite = c[2];
+  if (ovf) return fail(SR_EOVERFLOW, "traversal stack overflow in the last render");
+  return 0;
+}
+
+int sr_last_kernel_ms(sr_handle* h, float* ms) {
+  if (!h || !ms) return fail(SR_EINVAL, "null argument");
+  if (!h->timed) return fail(SR_ESTATE, "no render recorded");
+  HIPCHK(hipEventSynchronize(h->ev1));
+  HIPCHK(hipEventElapsedTime(ms, h->ev0, h->ev1));
+  return 0;
+}
+
+int sr_sync(sr_handle* h) {
+  if (!h) return fail(SR_EINVAL, "null handle");
+  HIPCHK(hipStreamSynchronize(h->st));
+  return 0;
+}
+
+}  // extern "C"

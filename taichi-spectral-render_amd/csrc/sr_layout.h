@@ -1,0 +1,48 @@
+// sr_layout.h -- device-memory layout of a compiled scene (host and device).
+//
+// HBM layout (all arrays 16-byte aligned, one allocation each):
+//   nodes   Node4[n_nodes]      128 B lines: SoA child boxes + child refs
+//   tris    TriRec[n_tris]      48 B: v0|prim, e1, e2 in BVH leaf order
+//   shade   ShadeRec[n_tris]    64 B: face normal|area, n0|mat, n1, n2,
+//                               indexed by ORIGINAL face id (fetched once
+//                               per closest hit, never during traversal)
+//   mats    MatRec[M][B]        32 B per (material, band)
+//   lights  LightRec[B][Lmax]   per-band area CDF over emissive faces
+//   lverts  float4[n_slots][3]  vertices of emissive faces (light slots)
+#pragma once
+#include <stdint.h>
+
+#define SR_STACK_SIZE 40     // per-lane LDS traversal stack entries
+#define SR_MAX_BVH4_DEPTH 13 // builder guarantee: 3 * depth + 1 <= stack
+#define SR_LEAF_MAX 4        // triangles per leaf
+#define SR_MAX_BANDS 64
+
+// Child reference in Node4::child: >= 0 is an inner node index; with the
+// sign bit set it is a leaf: bits 3..30 = first TriRec, bits 0..2 = count-1.
+// An empty slot has child = SR_EMPTY and an inverted (never-hit) box.
+#define SR_EMPTY ((int)0x80000000)
+#define SR_LEAF_BIT 0x80000000u
+
+struct alignas(16) Node4 {
+  float lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4], hi_z[4];
+  int child[4];
+  int pad[4];
+};
+static_assert(sizeof(Node4) == 128, "Node4 must be one 128-byte line");
+
+struct alignas(16) MatRec {  // one (material, band) of get_mat, main.py:980-996
+  float n, k, r, ss, em, g;
+  int tv, se;
+};
+static_assert(sizeof(MatRec) == 32, "MatRec is 32 bytes");
+
+struct alignas(16) LightRec {
+  float cdf;
+  int prim;  // original face id
+  int slot;  // index into lverts
+  int pad;
+};
+
+static inline int sr_leaf_ref(uint32_t first, uint32_t count) {
+  return (int)(SR_LEAF_BIT | (first << 3) | (count - 1u));
+}

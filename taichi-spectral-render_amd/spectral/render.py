@@ -1,0 +1,184 @@
+"""Host-side mirror of the reference's render protocol on the HIP library.
+
+Xyzonox/taichi-spectral-render drives its kernels from a GUI loop
+(main.py:1826-1837): ``cast_rays(band, block)`` once per band,
+``average_and_reset_block(n)`` and ``color_rays()`` once per block of B
+launches, ``clear_pixel_space()`` and ``update_camera()`` on a camera change
+(main.py:1812-1823).  ``Renderer`` keeps those operations and names, but one
+``render(n_blocks)`` call runs whole blocks (all B bands of every pixel) in a
+single persistent launch and folds them into the running mean.
+
+Device memory is held as torch tensors on the current HIP device; torch is
+only the allocator and stream provider here.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .scene import FlatScene, flatten_scene, pixel_center_rays, scene_dimensions
+
+
+def _stream_ptr():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Renderer:
+    """One scene, one accumulator.
+
+    ``scene`` is a reference-style scene dict or a ``FlatScene``.
+    """
+
+    def __init__(self, scene, spectral_bands, wavelength_min, wavelength_max, width=None, height=None,
+                 device="cuda"):
+        self.bands = int(spectral_bands)
+        self.wl_min = float(wavelength_min)
+        self.wl_max = float(wavelength_max)
+        self.wavelength_step = (self.wl_max - self.wl_min) / (self.bands - 1)
+        self.device = torch.device(device)
+        if isinstance(scene, FlatScene):
+            self.flat = scene
+        else:
+            self.dims = scene_dimensions(scene, self.bands)
+            self.flat = flatten_scene(scene, self.bands, self.wl_min, self.wl_max)
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.call("sr_create", self.bands, self.wl_min, self.wl_max, ctypes.byref(h))
+        self._h = h
+        self._upload(self.flat)
+        self.accum = None
+        self.width = self.height = 0
+        self.blocks_done = 0
+        if width is not None:
+            self.set_resolution(width, height if height is not None else width)
+
+    # ------------------------------------------------------------ lifecycle
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib.load().sr_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _bind_stream(self):
+        _lib.call("sr_set_stream", self._h, _stream_ptr())
+
+    def _upload(self, f: FlatScene):
+        self._bind_stream()
+        _lib.call("sr_upload_scene", self._h, _lib.ptr(f.tris_v), _lib.ptr(f.tris_n), _lib.ptr(f.tri_mat), f.n_tris,
+                  _lib.ptr(f.mat_f32), _lib.ptr(f.mat_i32), _lib.ptr(f.mat_eem), f.n_mats, _lib.ptr(f.eem),
+                  f.n_eem, _lib.ptr(f.cam))
+
+    def update_materials(self, mat_f32, mat_i32=None, mat_eem=None, eem=None):
+        """Replace material tables (same shapes) without rebuilding the BVH."""
+        f = self.flat
+        mf = np.ascontiguousarray(mat_f32, np.float32)
+        mi = np.ascontiguousarray(f.mat_i32 if mat_i32 is None else mat_i32, np.int32)
+        me = np.ascontiguousarray(f.mat_eem if mat_eem is None else mat_eem, np.int32)
+        et = np.ascontiguousarray(f.eem if eem is None else eem, np.float32)
+        self._bind_stream()
+        _lib.call("sr_update_materials", self._h, _lib.ptr(mf), _lib.ptr(mi), _lib.ptr(me), mf.shape[0],
+                  _lib.ptr(et), et.shape[0])
+        self.flat.mat_f32, self.flat.mat_i32, self.flat.mat_eem, self.flat.eem = mf, mi, me, et
+
+    def scene_info(self):
+        n = ctypes.c_int()
+        d = ctypes.c_int()
+        m = ctypes.c_int()
+        b = ctypes.c_longlong()
+        _lib.call("sr_scene_info", self._h, ctypes.byref(n), ctypes.byref(d), ctypes.byref(m), ctypes.byref(b))
+        return {"bvh_nodes": n.value, "bvh4_depth": d.value, "max_lights_per_band": m.value,
+                "device_bytes": b.value}
+
+    # ------------------------------------------------------------ reference protocol
+    def set_resolution(self, width, height):
+        self.width, self.height = int(width), int(height)
+        self.accum = torch.zeros((self.width, self.height, self.bands), dtype=torch.float32, device=self.device)
+        self.blocks_done = 0
+
+    def clear_pixel_space(self):
+        """clear_pixel_space + counter reset (main.py:1661-1674, 1819-1822)."""
+        if self.accum is not None:
+            self.accum.zero_()
+        self.blocks_done = 0
+
+    def update_camera(self, origin, lookat, up):
+        """update_camera (main.py:633-640) followed by the reset the reference
+        performs on every camera change (main.py:1816-1822)."""
+        cam = np.array(list(origin) + list(lookat) + list(up), np.float32)
+        _lib.call("sr_set_camera", self._h, _lib.ptr(cam))
+        self.flat.cam = cam
+        self.clear_pixel_space()
+
+    def render(self, n_blocks=1, max_depth=7, seed=0, stream=0):
+        """Run blocks [blocks_done, blocks_done + n_blocks) and fold them into
+        the running spectral mean ``accum`` [W, H, B] (cast_rays x B +
+        average_and_reset_block per block, main.py:1826-1836)."""
+        if self.accum is None:
+            raise RuntimeError("set_resolution() first")
+        self._bind_stream()
+        _lib.call("sr_render", self._h, self.width, self.height, int(seed), int(stream), self.blocks_done,
+                  int(n_blocks), int(max_depth), 0, _lib.ptr(self.accum))
+        self.blocks_done += int(n_blocks)
+        return self.accum
+
+    def color_rays(self, colour_system, exposure=1.0, gamma=2.0, out_width=None, out_height=None):
+        """color_rays (main.py:1609-1629) -> [out_w, out_h, 3] tensor."""
+        ow = self.width if out_width is None else int(out_width)
+        oh = self.height if out_height is None else int(out_height)
+        cmf = np.ascontiguousarray(colour_system.cmf, np.float32)
+        T = np.ascontiguousarray(colour_system.T, np.float32)
+        norm = float(np.float32(np.sum(colour_system.cmf[:, 1]) * self.wavelength_step))
+        out = torch.empty((ow, oh, 3), dtype=torch.float32, device=self.device)
+        self._bind_stream()
+        _lib.call("sr_resolve", self._h, _lib.ptr(self.accum), self.width, self.height, _lib.ptr(cmf), _lib.ptr(T),
+                  norm, float(exposure), float(gamma), ow, oh, _lib.ptr(out))
+        return out
+
+    # ------------------------------------------------------------ measurement / parity
+    def stats(self):
+        s = ctypes.c_uint64()
+        t = ctypes.c_uint64()
+        n = ctypes.c_uint64()
+        _lib.call("sr_stats", self._h, ctypes.byref(s), ctypes.byref(t), ctypes.byref(n))
+        return {"segments": s.value, "traversals": t.value, "nonfinite": n.value}
+
+    def last_kernel_ms(self):
+        ms = ctypes.c_float()
+        _lib.call("sr_last_kernel_ms", self._h, ctypes.byref(ms))
+        return ms.value
+
+    def sync(self):
+        _lib.call("sr_sync", self._h)
+
+    def trace(self, orig, dirs):
+        """Closest hit for rays (numpy [n,3] f32 host arrays or cuda tensors).
+        Returns (t, prim, uv) of the same kind."""
+        if isinstance(orig, torch.Tensor):
+            n = orig.shape[0]
+            t = torch.empty((n,), dtype=torch.float32, device=orig.device)
+            p = torch.empty((n,), dtype=torch.int32, device=orig.device)
+            uv = torch.empty((n, 2), dtype=torch.float32, device=orig.device)
+            self._bind_stream()
+            _lib.call("sr_trace_closest", self._h, n, _lib.ptr(orig.contiguous()), _lib.ptr(dirs.contiguous()),
+                      _lib.ptr(t), _lib.ptr(p), _lib.ptr(uv), _lib.SR_DEVICE_PTRS)
+            return t, p, uv
+        o = np.ascontiguousarray(orig, np.float32)
+        d = np.ascontiguousarray(dirs, np.float32)
+        n = o.shape[0]
+        t = np.empty((n,), np.float32)
+        p = np.empty((n,), np.int32)
+        uv = np.empty((n, 2), np.float32)
+        self._bind_stream()
+        _lib.call("sr_trace_closest", self._h, n, _lib.ptr(o), _lib.ptr(d), _lib.ptr(t), _lib.ptr(p), _lib.ptr(uv), 0)
+        return t, p, uv
+
+    def primary_rays(self, width=None, height=None):
+        return pixel_center_rays(self.flat, width or self.width, height or self.height)
