@@ -105,6 +105,23 @@ int sr_trace_closest(sr_handle* h, int n, const float* orig, const float* dir, f
 int sr_render(sr_handle* h, int width, int height, uint64_t seed, uint32_t stream, int first_block, int n_blocks,
               int max_depth, int flags, float* dev_accum);
 
+/* Path-replay adjoint of sr_render (no reference counterpart: the reference
+ * has no autodiff).  Replays blocks [first_block, first_block + n_blocks)
+ * with the same seed/stream as the forward call and ADDS to dev_grad
+ * [n_params] the gradient of L = sum(dev_d_accum * accum), where accum is
+ * the running mean after those blocks (so each block weighs
+ * 1 / (first_block + n_blocks)).  Parameters:
+ *   albedo  per band, for every material m with alb_off[m] >= 0 (host array
+ *           [n_mats]); param index alb_off[m] + band; the albedo is
+ *           s / (s + 4 pi k / lambda + 1e-6) of main.py:1070, 1395;
+ *   roughness of material rough_mat (-1: none) at param index rough_idx;
+ *   anistropy_factor g of material g_mat at g_idx;
+ *   sigma_a = 4 pi k / lambda per band of material sig_mat at sig_off + band.
+ * n_params <= 1024. */
+int sr_render_adjoint(sr_handle* h, int width, int height, uint64_t seed, uint32_t stream, int first_block,
+                      int n_blocks, int max_depth, const float* dev_d_accum, const int* alb_off, int rough_mat,
+                      int rough_idx, int g_mat, int g_idx, int sig_mat, int sig_off, int n_params, float* dev_grad);
+
 /* color_rays (main.py:1609-1629): dev_accum [W][H][B] -> dev_rgb
  * [out_w][out_h][3] with nearest upsampling.  cmf [B][3] (host),
  * xyz2rgb [3][3] row-major (host), norm = sum(cmf_y) * step. */
@@ -116,7 +133,7 @@ int sr_resolve(sr_handle* h, const float* dev_accum, int width, int height, cons
  * (extension + shadow), dropped non-finite contributions. */
 int sr_stats(sr_handle* h, uint64_t* segments, uint64_t* traversals, uint64_t* nonfinite);
 
-/* Device time of the last sr_render's path kernel in ms (blocking; HIP
+/* Device time of the last sr_render's (or sr_render_adjoint's) path kernel in ms (blocking; HIP
  * events recorded on the handle's stream around the launch). */
 int sr_last_kernel_ms(sr_handle* h, float* ms);
 

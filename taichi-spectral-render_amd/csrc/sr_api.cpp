@@ -87,8 +87,8 @@ struct sr_handle {
   std::vector<int> h_light_count;
   float cam_o[3], cam_u[3], cam_v[3], cam_w[3];
   // work
-  DevBuf sum, counters, queue, overflow, tmp;
-  int grid = 0;
+  DevBuf sum, counters, queue, overflow, tmp, alb_off;
+  int grid = 0, adj_grid = 0;
 };
 
 static void fill_scene_dev(const sr_handle* h, SceneDev& s) {
@@ -189,7 +189,7 @@ int sr_create(int bands, float wl_min, float wl_max, sr_handle** out) {
     return fail(SR_EHIP, "HIP init failed: %s", hipGetErrorString(e));
   }
   h->st = h->own;
-  if (sr_path_grid(&h->grid) != 0) {
+  if (sr_path_grid(&h->grid) != 0 || sr_adjoint_grid(&h->adj_grid) != 0) {
     delete h;
     return fail(SR_EHIP, "occupancy query failed");
   }
@@ -212,7 +212,7 @@ int sr_destroy(sr_handle* h) {
   if (h->st) (void)hipStreamSynchronize(h->st);
   DevBuf* bufs[] = {&h->nodes, &h->tris, &h->shade, &h->mats, &h->mat_eem, &h->eem, &h->lights, &h->lverts,
                     &h->light_count, &h->light_area, &h->lam_m, &h->lam_nm, &h->sum, &h->counters, &h->queue,
-                    &h->overflow, &h->tmp};
+                    &h->overflow, &h->tmp, &h->alb_off};
   for (DevBuf* b : bufs) b->release();
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -441,22 +441,21 @@ int sr_trace_closest(sr_handle* h, int n, const float* orig, const float* dir, f
   return 0;
 }
 
-int sr_render(sr_handle* h, int width, int height, uint64_t seed, uint32_t stream, int first_block, int n_blocks,
-              int max_depth, int flags, float* dev_accum) {
-  g_err.clear();
-  (void)flags;
+static int check_render_args(sr_handle* h, int width, int height, int first_block, int n_blocks, int max_depth,
+                             uint32_t stream) {
   if (!h) return fail(SR_EINVAL, "null handle");
   if (!h->has_scene) return fail(SR_ESTATE, "no scene uploaded");
   if (width <= 0 || height <= 0) return fail(SR_EINVAL, "bad resolution %dx%d", width, height);
   if (first_block < 0 || n_blocks < 0) return fail(SR_EINVAL, "bad block range");
   if (max_depth < 0) return fail(SR_EINVAL, "max_depth < 0");
   if (stream > 0xFFFFFFu) return fail(SR_EINVAL, "stream id must fit 24 bits");
-  if (!dev_accum) return fail(SR_EINVAL, "dev_accum is NULL");
   const unsigned long long items = (unsigned long long)width * height * (unsigned long long)n_blocks;
   if (items >= 0x7FFFFFFFull) return fail(SR_EINVAL, "W*H*n_blocks must be < 2^31 per call");
-  if ((unsigned long long)width * height >= (1ull << 31)) return fail(SR_EINVAL, "too many pixels");
-  HIPCHK(hipSetDevice(h->device));
-  if (n_blocks == 0) return 0;
+  return 0;
+}
+
+static int prepare_path_args(sr_handle* h, PathArgs& a, int width, int height, uint64_t seed, uint32_t stream,
+                             int first_block, int n_blocks, int max_depth) {
   hipStream_t st = h->st;
   const size_t n = (size_t)width * height * h->B;
   if (h->sum.bytes < n * 4) {
@@ -467,13 +466,13 @@ int sr_render(sr_handle* h, int width, int height, uint64_t seed, uint32_t strea
   HIPCHK(hipMemsetAsync(h->counters.p, 0, 4 * sizeof(unsigned long long), st));
   HIPCHK(hipMemsetAsync(h->queue.p, 0, 4, st));
   HIPCHK(hipMemsetAsync(h->overflow.p, 0, 4, st));
-  PathArgs a;
+  std::memset(&a, 0, sizeof(a));
   fill_scene_dev(h, a.s);
   a.sum = (float*)h->sum.p;
   a.counters = (unsigned long long*)h->counters.p;
   a.queue = (uint32_t*)h->queue.p;
   a.overflow = (uint32_t*)h->overflow.p;
-  a.n_items = (uint32_t)items;
+  a.n_items = (uint32_t)((unsigned long long)width * height * (unsigned long long)n_blocks);
   a.W = width;
   a.H = height;
   a.depth = max_depth;
@@ -483,15 +482,77 @@ int sr_render(sr_handle* h, int width, int height, uint64_t seed, uint32_t strea
   a.seed_lo = (uint32_t)seed;
   a.seed_hi = (uint32_t)(seed >> 32);
   a.stream = stream;
+  a.adj.rough_mat = a.adj.g_mat = a.adj.sig_mat = -1;
+  a.adj.rough_idx = a.adj.g_idx = a.adj.sig_off = -1;
+  return 0;
+}
+
+int sr_render(sr_handle* h, int width, int height, uint64_t seed, uint32_t stream, int first_block, int n_blocks,
+              int max_depth, int flags, float* dev_accum) {
+  g_err.clear();
+  (void)flags;
+  int rc = check_render_args(h, width, height, first_block, n_blocks, max_depth, stream);
+  if (rc) return rc;
+  if (!dev_accum) return fail(SR_EINVAL, "dev_accum is NULL");
+  HIPCHK(hipSetDevice(h->device));
+  if (n_blocks == 0) return 0;
+  PathArgs a;
+  if ((rc = prepare_path_args(h, a, width, height, seed, stream, first_block, n_blocks, max_depth))) return rc;
+  hipStream_t st = h->st;
   int grid = h->grid;
-  const unsigned long long waves_needed = (items + 255) / 256;
+  const unsigned long long waves_needed = ((unsigned long long)a.n_items + 255) / 256;
   if ((unsigned long long)grid > waves_needed) grid = (int)waves_needed;
   HIPCHK(hipEventRecord(h->ev0, st));
   if (sr_launch_path(a, grid, st)) return fail(SR_EHIP, "path kernel launch failed");
   HIPCHK(hipEventRecord(h->ev1, st));
   h->timed = true;
+  const size_t n = (size_t)width * height * h->B;
   if (sr_launch_finalize(dev_accum, (float*)h->sum.p, n, first_block, n_blocks, st))
     return fail(SR_EHIP, "finalize launch failed");
+  return 0;
+}
+
+int sr_render_adjoint(sr_handle* h, int width, int height, uint64_t seed, uint32_t stream, int first_block,
+                      int n_blocks, int max_depth, const float* dev_d_accum, const int* alb_off, int rough_mat,
+                      int rough_idx, int g_mat, int g_idx, int sig_mat, int sig_off, int n_params, float* dev_grad) {
+  g_err.clear();
+  int rc = check_render_args(h, width, height, first_block, n_blocks, max_depth, stream);
+  if (rc) return rc;
+  if (!dev_d_accum || !dev_grad || !alb_off) return fail(SR_EINVAL, "null array");
+  if (n_params <= 0 || n_params > SR_MAX_PARAMS) return fail(SR_EINVAL, "n_params=%d outside [1, %d]", n_params,
+                                                              SR_MAX_PARAMS);
+  const int B = h->B, M = h->n_mats;
+  for (int m = 0; m < M; ++m)
+    if (alb_off[m] >= 0 && alb_off[m] + B > n_params) return fail(SR_EINVAL, "alb_off[%d] out of range", m);
+  if (rough_mat >= M || g_mat >= M || sig_mat >= M) return fail(SR_EINVAL, "parameter material out of range");
+  if (rough_mat >= 0 && (rough_idx < 0 || rough_idx >= n_params)) return fail(SR_EINVAL, "bad rough_idx");
+  if (g_mat >= 0 && (g_idx < 0 || g_idx >= n_params)) return fail(SR_EINVAL, "bad g_idx");
+  if (sig_mat >= 0 && (sig_off < 0 || sig_off + B > n_params)) return fail(SR_EINVAL, "bad sig_off");
+  HIPCHK(hipSetDevice(h->device));
+  if (n_blocks == 0) return 0;
+  PathArgs a;
+  if ((rc = prepare_path_args(h, a, width, height, seed, stream, first_block, n_blocks, max_depth))) return rc;
+  if ((rc = h->alb_off.upload(alb_off, (size_t)M, h->st))) return rc;
+  a.adj.d_accum = dev_d_accum;
+  a.adj.grad = dev_grad;
+  a.adj.alb_off = (const int*)h->alb_off.p;
+  a.adj.rough_mat = rough_mat < 0 ? -2 : rough_mat;
+  a.adj.rough_idx = rough_idx;
+  a.adj.g_mat = g_mat < 0 ? -2 : g_mat;
+  a.adj.g_idx = g_idx;
+  a.adj.sig_mat = sig_mat < 0 ? -2 : sig_mat;
+  a.adj.sig_off = sig_off;
+  a.adj.n_params = n_params;
+  a.adj.scale = 1.0f / (float)(first_block + n_blocks);
+  hipStream_t st = h->st;
+  int grid = h->adj_grid;
+  const unsigned long long waves_needed = ((unsigned long long)a.n_items + 255) / 256;
+  if ((unsigned long long)grid > waves_needed) grid = (int)waves_needed;
+  HIPCHK(hipEventRecord(h->ev0, st));
+  if (sr_launch_adjoint(a, grid, st)) return fail(SR_EHIP, "adjoint kernel launch failed");
+  HIPCHK(hipEventRecord(h->ev1, st));
+  h->timed = true;
+  HIPCHK(hipStreamSynchronize(st));  // alb_off staging buffer is reused by the next call
   return 0;
 }
 

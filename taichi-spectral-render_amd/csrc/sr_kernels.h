@@ -6,6 +6,20 @@
 
 #include "sr_device.h"
 
+#define SR_MAX_PARAMS 1024
+
+// Differentiable parameters of the adjoint (sr_render_adjoint): per-band
+// albedo of any material with alb_off[m] >= 0 (param index alb_off[m] + band),
+// the roughness of one material, the HG g of one material and the per-band
+// sigma_a of one material (param index sig_off + band).
+struct AdjMap {
+  const float* d_accum;  // [W][H][B] dL / d accum
+  float* grad;           // [n_params], accumulated into
+  const int* alb_off;    // [M] device array
+  int rough_mat, rough_idx, g_mat, g_idx, sig_mat, sig_off, n_params;
+  float scale;           // 1 / (first_block + n_blocks)
+};
+
 struct PathArgs {
   SceneDev s;
   float* sum;                     // [W][H][B] block sums of this launch (zero on entry)
@@ -15,6 +29,7 @@ struct PathArgs {
   uint32_t n_items;               // W * H * n_blocks
   int W, H, depth, first_block, n_blocks, tiled;
   uint32_t seed_lo, seed_hi, stream;
+  AdjMap adj;
 };
 
 struct ResolveParams {
@@ -26,7 +41,9 @@ struct ResolveParams {
 int sr_launch_trace(const SceneDev& s, int n, const float* o, const float* d, float* t, int* prim, float* uv,
                     uint32_t* overflow, hipStream_t st);
 int sr_path_grid(int* grid_out);
+int sr_adjoint_grid(int* grid_out);
 int sr_launch_path(const PathArgs& a, int grid, hipStream_t st);
+int sr_launch_adjoint(const PathArgs& a, int grid, hipStream_t st);
 int sr_launch_finalize(float* mean, float* sum, size_t n, int first, int nb, hipStream_t st);
 int sr_launch_fill(float* p, size_t n, float v, hipStream_t st);
 int sr_launch_resolve(const float* acc, int W, int H, int B, const ResolveParams& p, float* out, int OW, int OH,

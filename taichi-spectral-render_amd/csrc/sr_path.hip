@@ -1,0 +1,769 @@
+// sr_path.hip -- the persistent spectral path loop (forward) and its
+// path-replay adjoint, sharing one lane state machine.
+//
+// Work item = (pixel, spectral block).  A lane runs the item's B single-band
+// paths one after another (cast_rays once per band, main.py:1502-1590),
+// fetching items from a device queue in wave-sized batches.  Every loop
+// iteration performs exactly one closest-hit traversal per active lane --
+// either a path's extension ray (one path segment) or one step of its NEE
+// shadow walk -- so lanes of a wave stay converged in the traversal, the
+// dominant cost.  Per-lane state lives in VGPRs; the traversal stack in LDS.
+//
+// Adjoint (ADJ = true): each path is replayed twice with identical random
+// numbers.  Pass 0 accumulates L = sum_k dL/dacc[band_k] * c_k over the
+// path's contributions c_k.  Pass 1 repeats the path and, at every
+// parameter-dependent step, adds to the gradient
+//   (radiance still to come) * (d log factor / d theta + d log p / d theta)
+// for throughput factors and sampling densities (score function), plus
+// dL/dacc * d c_k / d theta for the local terms of each contribution
+// (NEE bsdf/phase, MIS weights, shadow transmittance).  Derivatives of the
+// local terms come from dual numbers (sr_dual.h); values and branch
+// decisions always come from the forward code, so both passes follow the
+// forward path exactly.
+#include <hip/hip_runtime.h>
+
+#include "sr_dual.h"
+#include "sr_kernels.h"
+
+enum { PH_NEW = 0, PH_EXT = 1, PH_SHD = 2, PH_DONE = 3, PH_REPLAY = 4 };
+
+struct Lane {
+  int phase, band, pi, pj, blk;
+  Rng rng;
+  f3 pos, dir;
+  float tp, sp;
+  int active, pl, vol, seg;
+  float acc;
+  int acc_band;
+  // pending extension event (scatter_volume result)
+  f3 en, esn, edir;
+  int efront, emat, eaction, enew, nee_surf;
+  // pending shadow walk (shadow_transmittance state)
+  f3 sdir;
+  float smax, strav, stot, soff;
+  int sprim, svol, sstep;
+  uint32_t n_seg, n_trav, n_nf;
+  // adjoint only
+  int pass;
+  float Lw, Eb, sp_dr, stot_dsig;
+};
+
+template <bool ADJ>
+struct Ctx {
+  const PathArgs& a;
+  float* lds_grad;
+};
+
+__device__ __forceinline__ float dacc(const PathArgs& a, const Lane& L, int band) {
+  return a.adj.d_accum[((size_t)L.pi * a.H + L.pj) * a.s.B + band];
+}
+
+template <bool ADJ>
+__device__ __forceinline__ void emit_grad(const Ctx<ADJ>& c, int idx, float v) {
+  if (idx >= 0 && v != 0.0f && isfinite(v)) atomicAdd(c.lds_grad + idx, v);
+}
+
+template <bool ADJ>
+__device__ __forceinline__ void flush_acc(Lane& L, const Ctx<ADJ>& c) {
+  if (!ADJ && L.acc != 0.0f) {
+    size_t idx = ((size_t)L.pi * c.a.H + L.pj) * c.a.s.B + L.acc_band;
+    atomicAdd(c.a.sum + idx, L.acc);
+  }
+  L.acc = 0.0f;
+}
+
+// block_accumulator[i, j, active] += contribution (main.py:1589-1590);
+// non-finite contributions are dropped and counted.
+template <bool ADJ>
+__device__ __forceinline__ void add_contrib(Lane& L, const Ctx<ADJ>& c, float v) {
+  if (!isfinite(v)) {
+    L.n_nf++;
+    return;
+  }
+  if (v == 0.0f) return;
+  if (ADJ) {
+    float w = dacc(c.a, L, L.active) * v;
+    if (L.pass == 0) L.Lw += w;
+    else L.Eb += w;
+    return;
+  }
+  if (L.active != L.acc_band) {
+    flush_acc<ADJ>(L, c);
+    L.acc_band = L.active;
+  }
+  L.acc += v;
+}
+
+template <bool ADJ>
+__device__ __forceinline__ void end_path(Lane& L, const Ctx<ADJ>& c) {
+  flush_acc<ADJ>(L, c);
+  if (ADJ && L.pass == 0) {
+    L.pass = 1;  // replay the same path for the gradient pass
+    L.phase = PH_REPLAY;
+  } else {
+    L.phase = PH_NEW;
+  }
+}
+
+// Camera ray of cast_rays, main.py:1506-1529.
+template <bool ADJ>
+__device__ __forceinline__ void start_path(Lane& L, const Ctx<ADJ>& c) {
+  const PathArgs& a = c.a;
+  const SceneDev& s = a.s;
+  L.rng.init(a.seed_lo, a.seed_hi, (uint32_t)(L.pi * a.H + L.pj), (uint32_t)L.blk,
+             (uint32_t)L.band | (a.stream << 8));
+  float u = ((float)L.pi + L.rng.next()) / (float)a.W;
+  float v = ((float)L.pj + L.rng.next()) / (float)a.H;
+  const float aspect = (float)a.W / (float)a.H;
+  f3 cu = mk(s.cam_u[0], s.cam_u[1], s.cam_u[2]);
+  f3 cv = mk(s.cam_v[0], s.cam_v[1], s.cam_v[2]);
+  f3 cw = mk(s.cam_w[0], s.cam_w[1], s.cam_w[2]);
+  L.dir = normalize(cu * (u * 2.0f - 1.0f) * aspect * 1.0f + cv * (v * 2.0f - 1.0f) * 1.0f - cw * 2.0f);
+  L.pos = mk(s.cam_o[0], s.cam_o[1], s.cam_o[2]);
+  L.tp = 1.0f;
+  L.sp = 1.0f;
+  L.active = L.band;
+  L.pl = 0;
+  L.vol = 0;
+  L.seg = 0;
+  L.acc = 0.0f;
+  L.acc_band = L.band;
+  if (ADJ) {
+    if (L.pass == 0) L.Lw = 0.0f;
+    L.Eb = 0.0f;
+    L.sp_dr = 0.0f;
+  }
+  if (a.depth > 0) {
+    L.phase = PH_EXT;
+  } else {
+    end_path<ADJ>(L, c);
+    if (L.phase == PH_REPLAY) L.phase = PH_NEW;
+  }
+}
+
+// evaluate_bsdf, main.py:1360-1444 (its transmission branch is unreachable:
+// both cosines are clamped to >= 1e-4 before NdotV * NdotL > 0 is tested).
+__device__ __forceinline__ void evaluate_bsdf(const SceneDev& s, int vol_id, int surf_id, f3 V, f3 Lv, f3 N, int b,
+                                              float& bsdf, float& pdf) {
+  MatRec vm = get_mat(s, vol_id, b);
+  MatRec sm = get_mat(s, surf_id, b);
+  float NdotL = tmax(1e-4f, dot(N, Lv));
+  float NdotV = tmax(1e-4f, dot(N, V));
+  bsdf = 0.0f;
+  pdf = 0.0f;
+  f3 H = normalize(V + Lv);
+  if (dot(H, N) < 0.0f) H = -H;
+  float VdotH = tmax(1e-4f, dot(V, H));
+  float F = fresnel_spectral(VdotH, vm.n, vm.k, sm.n, sm.k);
+  if (sm.r > 0.008f) {
+    float NdotH = tmax(1e-4f, dot(N, H));
+    float D = ggx_D(NdotH, sm.r);
+    float G = ggx_G(N, V, Lv, sm.r);
+    bsdf += (D * G * F) / (4.0f * NdotL * NdotV);
+    pdf += (D * NdotH * F) / (4.0f * VdotH);
+  }
+  float absorb = (4.0f * SR_PI * sm.k * 1e9f) / s.lam_nm[b];
+  float albedo = sm.ss / (sm.ss + absorb + 1e-6f);
+  float on = oren_nayar(sm.r, NdotL, NdotV, Lv, V, N);
+  float opq = 1.0f - (float)sm.tv;
+  bsdf += (1.0f - F) * opq * (albedo / SR_PI) * on;
+  pdf += NdotL * 0.318309886184f * (1.0f - F) * opq;
+}
+
+// Dual version (derivatives only) of evaluate_bsdf with respect to the
+// surface's albedo and, when it is the designated rough material, its
+// roughness.  The Fresnel term's dependence on k is not differentiated (k of
+// a diffuse albedo is ~1e-8 and moves F by less than 1e-7).
+__device__ __forceinline__ void evaluate_bsdf_d(const SceneDev& s, const AdjMap& m, int vol_id, int surf_id, f3 V,
+                                                f3 Lv, f3 N, int b, Dl& bsdf, Dl& pdf) {
+  MatRec vm = get_mat(s, vol_id, b);
+  MatRec sm = get_mat(s, surf_id, b);
+  Dl r = dvar(sm.r, SL_ROUGH, surf_id == m.rough_mat);
+  float NdotL = tmax(1e-4f, dot(N, Lv));
+  float NdotV = tmax(1e-4f, dot(N, V));
+  bsdf = dc(0.0f);
+  pdf = dc(0.0f);
+  f3 H = normalize(V + Lv);
+  if (dot(H, N) < 0.0f) H = -H;
+  float VdotH = tmax(1e-4f, dot(V, H));
+  float F = fresnel_spectral(VdotH, vm.n, vm.k, sm.n, sm.k);
+  if (sm.r > 0.008f) {
+    float NdotH = tmax(1e-4f, dot(N, H));
+    Dl D = ggx_D_d(NdotH, r);
+    Dl G = ggx_G_d(N, V, Lv, r);
+    bsdf = bsdf + (D * G * F) / (4.0f * NdotL * NdotV);
+    pdf = pdf + (D * (NdotH * F)) / (4.0f * VdotH);
+  }
+  float absorb = (4.0f * SR_PI * sm.k * 1e9f) / s.lam_nm[b];
+  Dl albedo = dvar(sm.ss / (sm.ss + absorb + 1e-6f), SL_ALB, m.alb_off[surf_id] >= 0);
+  Dl on = oren_nayar_d(r, NdotL, NdotV, Lv, V, N);
+  float opq = 1.0f - (float)sm.tv;
+  bsdf = bsdf + ((albedo * ((1.0f - F) * opq / SR_PI)) * on);
+  pdf = pdf + NdotL * 0.318309886184f * (1.0f - F) * opq;
+}
+
+// scatter_surface, main.py:1000-1188, applied to the lane's ray as
+// cast_rays does (main.py:1565-1574).  `direct` is this segment's NEE
+// contribution (accumulated after the scatter), needed by the adjoint.
+template <bool ADJ>
+__device__ __forceinline__ void scatter_surface(Lane& L, const Ctx<ADJ>& c, float direct) {
+  const PathArgs& a = c.a;
+  const SceneDev& s = a.s;
+  const int ab = L.active;
+  MatRec sm = get_mat(s, L.emat, ab);
+  MatRec vm = get_mat(s, L.vol, ab);
+  const f3 V = -L.dir;
+  const f3 N = L.esn;
+  const float ni = vm.n, ki = vm.k, nt = sm.n, kt = sm.k;
+  float x0 = L.rng.next();
+  float x1 = L.rng.next();
+  f3 H = ggx_sample_half(x0, x1, N, sm.r);
+  float ia = tmax(1e-4f, dot(V, H));
+  float F = fresnel_spectral(ia, ni, ki, nt, kt);
+  float rnd = clampf(L.rng.next(), 0.0f, 0.99f);
+  int dr = 0;
+  f3 ndir;
+  float est, pdf;
+  // branch id for the adjoint: 0 delta, 1 GGX reflection (max cos),
+  // 2 GGX reflection on TIR (|cos|), 3 Oren-Nayar, 4 rough refraction
+  int branch = 0;
+  float on_NL = 0.f, on_NV = 0.f, on_albedo = 0.f;
+  f3 on_S = mk(0.f, 0.f, 0.f);
+  float rf_LdotH = 0.f, rf_VdotH = 0.f, rf_NdotH = 0.f, rf_NdotL = 0.f, rf_NdotV = 0.f, rf_dsq = 0.f, rf_t1 = 0.f;
+  if (rnd < F) {
+    ndir = reflect(L.dir, H);
+    dr = 1;
+    if (sm.r < 0.008f) { est = F; pdf = 1e9f; }
+    else branch = 1;
+  } else {
+    float absorb = (4.0f * SR_PI * kt) / s.lam_m[ab];
+    float dte = -logf(-L.rng.next() + 1.0f + 1e-7f) / (sm.ss + absorb + 1e-6f);
+    if (!sm.tv) {
+      dr = 1;
+      branch = 3;
+      f3 S = cosine_hemisphere(N, L.rng);
+      float NdotL = tmax(1e-4f, dot(N, S));
+      float NdotV = tmax(1e-4f, dot(N, V));
+      float albedo = sm.ss / (sm.ss + absorb + 1e-6f);
+      float bsdf = (1.0f - F) * (albedo / SR_PI) * oren_nayar(sm.r, NdotL, NdotV, S, V, N);
+      pdf = 0.318309886184f * NdotL;
+      on_NL = NdotL; on_NV = NdotV; on_albedo = albedo; on_S = S;
+      const int eid = s.mat_eem[L.emat];
+      if (eid > 0) {  // surface fluorescence reweights only (main.py:1090-1121)
+        const float* row = s.eem + ((size_t)eid * s.B + ab) * s.B;
+        float qy = 0.0f;
+        for (int j = 0; j < s.B; ++j) qy = row[j] > qy ? row[j] : qy;
+        float r3 = L.rng.next();
+        if (qy > 0.0f && r3 < qy) {
+          float norm = 0.0f;
+          for (int j = 0; j < s.B; ++j) norm += row[j];
+          float r4 = L.rng.next();
+          float cum = 0.0f;
+          int nw = ab;
+          for (int j = 0; j < s.B; ++j) {
+            cum += row[j] / norm;
+            if (r4 < cum) { nw = j; break; }
+          }
+          MatRec fm = get_mat(s, L.emat, nw);
+          float ab2 = (4.0f * SR_PI * fm.k) / s.lam_m[nw];
+          bsdf *= expf(-ab2 * dte);
+        }
+      }
+      est = (bsdf * NdotL) / pdf;
+      ndir = S;
+    } else {
+      f3 Rf = refract(L.dir, H, ni / nt);
+      if (dot(Rf, Rf) < 1e-4f) {  // total internal reflection
+        ndir = reflect(L.dir, H);
+        dr = 1;
+        if (sm.r < 0.008f) { est = F; pdf = 1e9f; }
+        else branch = 2;
+      } else {
+        dr = 0;
+        ndir = Rf;
+        if (sm.r < 0.0018f) {
+          est = 1.0f - F;
+          pdf = 1e9f;
+        } else {
+          branch = 4;
+          float LdotH = dot(Rf, H);
+          float VdotH = dot(V, H);
+          float NdotH = tmax(1e-4f, dot(N, H));
+          float NdotL = tmax(1e-4f, fabsf(dot(N, Rf)));
+          float NdotV = tmax(1e-4f, fabsf(dot(N, V)));
+          float G = ggx_G(N, V, Rf, sm.r);
+          float D = ggx_D(NdotH, sm.r);
+          float den = ni * VdotH + nt * LdotH;
+          float dsq = den * den;
+          float t1 = fabsf(LdotH * VdotH) / (NdotL * NdotV);
+          float t2 = (nt * nt * (1.0f - F) * D * G) / dsq;
+          float bsdf = t1 * t2;
+          pdf = (D * NdotH * (1.0f - F) * nt * nt * fabsf(LdotH)) / dsq;
+          est = (bsdf * NdotL) / pdf;
+          rf_LdotH = LdotH; rf_VdotH = VdotH; rf_NdotH = NdotH; rf_NdotL = NdotL; rf_NdotV = NdotV;
+          rf_dsq = dsq; rf_t1 = t1;
+        }
+      }
+    }
+  }
+  float g_NdotL = 0.f, g_NdotH = 0.f, g_NdotV = 0.f, g_c = 0.f;
+  if (branch == 1 || branch == 2) {  // GGX reflection lobe (main.py:1039-1053 and 1142-1156)
+    float NdotL = tmax(1e-4f, dot(N, ndir));
+    float NdotH = tmax(1e-4f, dot(N, H));
+    float NdotV = tmax(1e-4f, dot(N, V));
+    float D = ggx_D(NdotH, sm.r);
+    float G = ggx_G(N, V, ndir, sm.r);
+    float bsdf = (D * G * F) / (4.0f * NdotL * NdotV);
+    pdf = (D * NdotH) / (4.0f * ia);
+    float cc = branch == 2 ? fabsf(dot(N, ndir)) : tmax(0.0f, dot(N, ndir));
+    est = (bsdf * cc) / pdf;
+    g_NdotL = NdotL; g_NdotH = NdotH; g_NdotV = NdotV; g_c = cc;
+  }
+  if (ADJ && L.pass == 1) {
+    const AdjMap& m = a.adj;
+    const bool rough = L.emat == m.rough_mat;
+    const int aoff = m.alb_off[L.emat];
+    // radiance that follows this scatter (excludes this segment's NEE term)
+    float R = L.Lw - L.Eb - (isfinite(direct) ? dacc(a, L, ab) * direct : 0.0f);
+    Dl r = dvar(sm.r, SL_ROUGH, rough);
+    Dl est_d = dc(est);
+    Dl pdf_d = dc(pdf);
+    if (branch == 1 || branch == 2) {
+      Dl D = ggx_D_d(g_NdotH, r);
+      Dl G = ggx_G_d(N, V, ndir, r);
+      pdf_d = (D * g_NdotH) / (4.0f * ia);
+      est_d = (((D * G) * F) / (4.0f * g_NdotL * g_NdotV) * g_c) / pdf_d;
+    } else if (branch == 3) {
+      Dl alb = dvar(on_albedo, SL_ALB, aoff >= 0);
+      Dl on = oren_nayar_d(r, on_NL, on_NV, on_S, V, N);
+      est_d = ((alb * ((1.0f - F) / SR_PI)) * on) * on_NL / pdf;
+    } else if (branch == 4) {
+      Dl D = ggx_D_d(rf_NdotH, r);
+      Dl G = ggx_G_d(N, V, ndir, r);
+      Dl t2 = (D * G) * (nt * nt * (1.0f - F)) / rf_dsq;
+      pdf_d = (D * (rf_NdotH * (1.0f - F) * nt * nt * fabsf(rf_LdotH))) / rf_dsq;
+      est_d = ((t2 * rf_t1) * rf_NdotL) / pdf_d;
+    }
+    float dl[SL_N];
+    dlog_into(est_d, dl);
+    if (aoff >= 0) emit_grad(c, aoff + ab, R * dl[SL_ALB]);
+    if (rough) {
+      // half vector drawn from D_{a2 = r^2}(h) (n.h) regardless of the branch
+      float sc = ggx_sample_score_r(tmax(1e-4f, dot(N, H)), sm.r);
+      emit_grad(c, m.rough_idx, R * (dl[SL_ROUGH] + sc));
+    }
+    L.sp_dr = rough ? pdf_d.d[SL_ROUGH] : 0.0f;
+  }
+  L.sp = pdf;
+  L.dir = ndir;
+  L.tp *= est;
+  float fdr = (float)dr;
+  L.pos = L.pos + L.en * SR_EPSILON * (fdr - (1.0f - fdr));
+  L.vol = L.emat * L.efront * (1 - dr) + L.vol * dr;
+}
+
+// End of one path-loop iteration (main.py:1565-1590).
+template <bool ADJ>
+__device__ __forceinline__ void finish_segment(Lane& L, const Ctx<ADJ>& c, float direct) {
+  if (L.eaction == 1) {
+    L.pl += 1;
+    scatter_surface<ADJ>(L, c, direct);
+  } else if (L.eaction == 2) {
+    L.dir = L.edir;
+    L.pl += 1;
+  } else if (L.eaction == 3) {
+    L.dir = L.edir;
+    L.active = L.enew;
+    L.pl += 1;
+  }
+  add_contrib<ADJ>(L, c, direct);
+  L.seg += 1;
+  if (L.seg >= c.a.depth) {
+    end_path<ADJ>(L, c);
+  } else if (!isfinite(L.tp)) {
+    // non-finite throughput (e.g. the 0/0 rough BTDF estimator on leaving a
+    // rough dielectric, main.py:1176-1185 with n_i == n_t): end the path
+    L.n_nf++;
+    end_path<ADJ>(L, c);
+  } else {
+    L.phase = PH_EXT;
+  }
+}
+
+// Extension ray came back: scatter_volume (main.py:1190-1292) and the
+// cast_rays event dispatch (main.py:1540-1578) up to the NEE shadow walk.
+template <bool ADJ>
+__device__ __forceinline__ void on_extension(Lane& L, const Ctx<ADJ>& c, const HitD& h) {
+  const PathArgs& a = c.a;
+  const SceneDev& s = a.s;
+  L.n_seg++;
+  L.n_trav++;
+  const int ab = L.active;
+  MatRec vm = get_mat(s, L.vol, ab);
+  float sa = (4.0f * SR_PI * vm.k) / s.lam_m[ab];
+  float dvol = -logf(-L.rng.next() + 1.0f - 1e-7f) / (vm.ss + sa + 1e-6f);
+  float ht = h.t;
+  f3 hn = mk(0.f, 0.f, 0.f), hsn = mk(0.f, 0.f, 0.f);
+  int hfront = 0, hmat = 0;
+  if (h.prim >= 0) {
+    SurfHit sh = surface_of(s, h, L.dir);
+    hn = sh.normal;
+    hsn = sh.sn;
+    hfront = sh.front;
+    hmat = sh.mat;
+  }
+  float travel = -1.0f;
+  if (ht < SR_ESCAPE) {
+    travel = ht;
+    if (dvol + SR_EPSILON < travel) travel = dvol;
+  }
+  const float ff = (float)hfront;
+  L.en = hn * ff - hn * (1.0f - ff);
+  L.esn = hsn * ff - hsn * (1.0f - ff);
+  L.efront = hfront;
+  float T = expf(-(sa + vm.ss) * travel);
+  float emit = ((-T + 1.0f) * vm.em) / (sa + 1e-6f);
+  float atten = T + emit;
+  float est = 0.0f;
+  float em_new = 0.0f;
+  bool absorb_branch = false;
+  L.eaction = 0;
+  L.emat = 0;
+  if (travel == ht) {
+    est = atten / (1.0f + 1e-6f);
+    L.eaction = 1;
+    L.emat = hmat;
+  } else if (travel >= 0.0f) {
+    L.emat = -1;
+    float re = L.rng.next();
+    float st = sa / (vm.ss + sa + 1e-6f);
+    if (re <= st) {
+      absorb_branch = true;
+      const int eid = s.mat_eem[L.vol];
+      if (eid > 0) {
+        const float* row = s.eem + ((size_t)eid * s.B + ab) * s.B;
+        float qy = 0.0f;
+        for (int j = 0; j < s.B; ++j) qy = row[j] > qy ? row[j] : qy;
+        float rf = L.rng.next();
+        if (qy > 0.0f && rf < qy) {
+          L.eaction = 3;
+          float norm = 0.0f;
+          for (int j = 0; j < s.B; ++j) norm += row[j];
+          float r2 = L.rng.next();
+          float cum = 0.0f;
+          int nw = ab;
+          for (int j = 0; j < s.B; ++j) {
+            cum += row[j] / norm;
+            if (r2 < cum) { nw = j; break; }
+          }
+          L.enew = nw;
+          L.edir = random_in_unit_sphere(L.rng);
+          em_new = get_mat(s, L.vol, nw).em;
+          float em2 = ((-(1.0f - T) + 1.0f) * em_new) / (sa + 1e-6f);
+          est = (1.0f - T) + em2;
+        } else {
+          est = atten;
+        }
+      } else {
+        est = atten / (1.0f + 1e-6f);
+      }
+    } else {
+      L.eaction = 2;
+      L.edir = hg_sample(L.dir, vm.g, L.rng);
+      est = atten;
+    }
+  } else {
+    end_path<ADJ>(L, c);  // escaped (main.py:1541-1543)
+    return;
+  }
+  if (ADJ && L.pass == 1) {
+    const AdjMap& m = a.adj;
+    const float R = L.Lw - L.Eb;  // everything from this segment on carries est
+    if (L.vol == m.sig_mat) {
+      Dl sad = dvar(sa, SL_SIG, true);
+      Dl Td = dexp((sad + vm.ss) * (-travel));
+      Dl attd = Td + ((1.0f - Td) * vm.em) / (sad + 1e-6f);
+      Dl estd;
+      const int eid = s.mat_eem[L.vol];
+      if (L.eaction == 1) estd = attd / (1.0f + 1e-6f);
+      else if (L.eaction == 3) estd = (1.0f - Td) + (Td * em_new) / (sad + 1e-6f);
+      else if (L.eaction == 2 || eid > 0) estd = attd;
+      else estd = attd / (1.0f + 1e-6f);
+      float dl[SL_N];
+      dlog_into(estd, dl);
+      const float st = vm.ss + sa + 1e-6f;
+      float score;
+      if (L.eaction == 1) {
+        score = -(ht - SR_EPSILON);  // P(no volume event before the surface)
+      } else {
+        score = 1.0f / st - travel;  // free-flight density
+        score += absorb_branch ? (sa > 0.f ? 1.0f / sa - 1.0f / st : 0.0f) : -1.0f / st;
+      }
+      emit_grad(c, m.sig_off + ab, R * (dl[SL_SIG] + score));
+    }
+    if (L.eaction == 2 && L.vol == m.g_mat) {
+      emit_grad(c, m.g_idx, R * hg_score_g(dot(L.edir, L.dir), vm.g));
+    }
+  }
+  L.pos = L.pos + L.dir * travel;
+  L.tp *= est;
+  int surf = -1;
+  if (L.eaction == 1) {
+    MatRec smt = get_mat(s, L.emat, L.band);
+    if (smt.se) {  // direct hit on an emitter (main.py:1550-1559)
+      float d2 = travel * travel;
+      float cl = tmax(0.0f, dot(hn, -L.dir));
+      float pdfl = (d2 * tmin(1.0f, (float)L.pl)) / (cl * s.light_area[L.active] + 1e-6f);
+      float den = pdfl * pdfl + L.sp * L.sp + 1e-6f;
+      float w = (L.sp * L.sp) / den;
+      float contrib = L.tp * smt.em * w;
+      if (ADJ && L.pass == 1 && L.sp_dr != 0.0f && isfinite(contrib)) {
+        float dlogw = 2.0f / L.sp - 2.0f * L.sp / den;
+        emit_grad(c, a.adj.rough_idx, dacc(a, L, L.active) * contrib * dlogw * L.sp_dr);
+      }
+      add_contrib<ADJ>(L, c, contrib);
+      end_path<ADJ>(L, c);
+      return;
+    }
+    surf = L.emat;
+  }
+  // sample_direct_light, main.py:1296-1326 (up to the shadow walk)
+  L.nee_surf = surf;
+  const int nl = s.light_count[ab];
+  if (nl <= 0) {
+    finish_segment<ADJ>(L, c, 0.0f);
+    return;
+  }
+  const LightRec* lr = s.lights + (size_t)ab * s.max_lights;
+  float r = L.rng.next();
+  int e = 0;
+  for (int i = 0; i < nl; ++i) {
+    if (r < lr[i].cdf) { e = i; break; }
+  }
+  LightRec le = lr[e];
+  const float4* lv = s.lverts + (size_t)le.slot * 3;
+  f3 v0 = xyz(lv[0]), v1 = xyz(lv[1]), v2 = xyz(lv[2]);
+  float bx = L.rng.next(), by = L.rng.next();
+  if (bx + by > 1.0f) { bx = 1.0f - bx; by = 1.0f - by; }
+  f3 p = v0 + bx * (v1 - v0) + by * (v2 - v0);
+  L.sdir = normalize(p - L.pos);
+  L.smax = length(p - L.pos);
+  L.sprim = le.prim;
+  L.svol = L.vol;
+  L.strav = 0.0f;
+  L.stot = 1.0f;
+  L.soff = 0.0f;
+  L.sstep = 0;
+  if (ADJ) L.stot_dsig = 0.0f;
+  L.phase = PH_SHD;
+}
+
+// Shadow ray came back: one step of shadow_transmittance (main.py:1456-1498);
+// on the last step, the rest of sample_direct_light (main.py:1329-1356).
+template <bool ADJ>
+__device__ __forceinline__ void on_shadow(Lane& L, const Ctx<ADJ>& c, const HitD& h) {
+  const PathArgs& a = c.a;
+  const SceneDev& s = a.s;
+  L.n_trav++;
+  const int ab = L.active;
+  MatRec vm = get_mat(s, L.svol, ab);
+  float absorb = (4.0f * SR_PI * vm.k) / s.lam_m[ab];
+  const bool sig_span = ADJ && L.svol == a.adj.sig_mat;
+  L.stot *= expf(-(absorb + vm.ss) * h.t);
+  if (sig_span) L.stot_dsig -= h.t;
+  bool done = false;
+  if (h.t >= (L.smax - L.strav) || h.prim == L.sprim) {
+    L.stot *= expf(-(absorb + vm.ss) * (L.smax - L.strav));
+    if (sig_span) L.stot_dsig -= (L.smax - L.strav);
+    done = true;
+  } else {
+    SurfHit sh = surface_of(s, h, L.sdir);
+    MatRec sm = get_mat(s, sh.mat, ab);
+    float n_i = vm.n;
+    float n_t = sh.front ? sm.n : get_mat(s, 0, ab).n;
+    const float ff = (float)sh.front;
+    f3 inc = sh.sn * ff - sh.sn * (1.0f - ff);
+    f3 refr = refract(L.sdir, inc, n_i / n_t);
+    float rt = (dot(L.sdir, refr) - 1.0f) * 37.9743f;
+    float af = 1.0f / (1.0f + rt * rt * rt * rt * (3.0f - 2.0f * rt));
+    L.stot *= af;
+    if (L.stot < 1e-6f || !sm.tv) {
+      L.stot = 0.0f;
+      done = true;
+    } else {
+      L.strav += h.t + SR_EPSILON;
+      L.soff = h.t + SR_EPSILON;
+      L.svol = sh.mat * sh.front;
+      L.sstep += 1;
+      if (L.sstep >= SR_SCATTER_BOUNCES) done = true;
+    }
+  }
+  if (!done) return;
+  float direct = 0.0f;
+  if (L.stot > 1e-6f) {
+    const float4* shl = s.shade + (size_t)L.sprim * 4;
+    f3 ln = xyz(shl[0]);
+    int lmat = __float_as_int(shl[1].w);
+    float cl = tmax(0.0f, dot(ln, -L.sdir));
+    float d2 = L.smax * L.smax;
+    float gt = cl / (d2 + 1e-4f);
+    float bsdf, cosv, pdf;
+    if (L.nee_surf != -1) {
+      cosv = tmax(0.0f, dot(L.esn, L.sdir));
+      evaluate_bsdf(s, L.vol, L.nee_surf, -L.dir, L.sdir, L.esn, ab, bsdf, pdf);
+    } else {
+      cosv = tmax(0.0f, dot(L.dir, L.sdir));
+      bsdf = hg_eval(cosv, get_mat(s, L.vol, ab).g);
+      pdf = bsdf;
+    }
+    float factor = bsdf * cosv * L.stot * gt * s.light_area[ab];
+    float emis = get_mat(s, lmat, L.band).em;
+    float pdfl = d2 / (tmax(1e-6f, cl) * s.light_area[ab]);
+    float w = (pdfl * pdfl) / (pdfl * pdfl + pdf * pdf + 1e-6f);
+    direct = L.tp * emis * factor * w;
+    if (ADJ && L.pass == 1 && isfinite(direct) && direct != 0.0f) {
+      const AdjMap& m = a.adj;
+      Dl bd, pd;
+      if (L.nee_surf != -1) {
+        evaluate_bsdf_d(s, m, L.vol, L.nee_surf, -L.dir, L.sdir, L.esn, ab, bd, pd);
+      } else {
+        bd = hg_eval_d(cosv, dvar(get_mat(s, L.vol, ab).g, SL_G, L.vol == m.g_mat));
+        pd = bd;
+      }
+      Dl wd = dc(pdfl * pdfl) / ((pd * pd) + (pdfl * pdfl + 1e-6f));
+      float lb[SL_N], lw[SL_N];
+      dlog_into(bd, lb);
+      dlog_into(wd, lw);
+      // accumulated into the band after this segment's update (main.py:1586-1590)
+      const int band_acc = L.eaction == 3 ? L.enew : ab;
+      const float dv = dacc(a, L, band_acc) * direct;
+      if (L.nee_surf != -1 && m.alb_off[L.nee_surf] >= 0)
+        emit_grad(c, m.alb_off[L.nee_surf] + ab, dv * (lb[SL_ALB] + lw[SL_ALB]));
+      if (L.nee_surf == m.rough_mat) emit_grad(c, m.rough_idx, dv * (lb[SL_ROUGH] + lw[SL_ROUGH]));
+      if (L.nee_surf == -1 && L.vol == m.g_mat) emit_grad(c, m.g_idx, dv * (lb[SL_G] + lw[SL_G]));
+      if (m.sig_mat >= 0) emit_grad(c, m.sig_off + ab, dv * L.stot_dsig);
+    }
+  }
+  finish_segment<ADJ>(L, c, direct);
+}
+
+__device__ __forceinline__ void item_to_pixel(const PathArgs& a, uint32_t item, int& pi, int& pj, int& blk) {
+  const uint32_t npix = (uint32_t)a.W * (uint32_t)a.H;
+  blk = a.first_block + (int)(item / npix);
+  uint32_t r = item % npix;
+  if (a.tiled) {
+    uint32_t tile = r >> 6, w = r & 63u;
+    uint32_t tx = tile % (uint32_t)(a.W >> 3), ty = tile / (uint32_t)(a.W >> 3);
+    pi = (int)(tx * 8 + (w & 7u));
+    pj = (int)(ty * 8 + (w >> 3));
+  } else {
+    pi = (int)(r / (uint32_t)a.H);
+    pj = (int)(r % (uint32_t)a.H);
+  }
+}
+
+template <bool ADJ>
+__device__ __forceinline__ void path_loop(const PathArgs& a, int* stk, float* lds_grad) {
+  const Ctx<ADJ> c{a, lds_grad};
+  const int stride = blockDim.x;
+  const int lane = (int)__lane_id();
+  Lane L;
+  L.phase = PH_NEW;
+  L.band = a.s.B;
+  L.pi = L.pj = L.blk = 0;
+  L.acc = 0.0f;
+  L.acc_band = 0;
+  L.n_seg = L.n_trav = L.n_nf = 0;
+  L.pass = 0;
+  L.Lw = L.Eb = L.sp_dr = L.stot_dsig = 0.0f;
+  while (true) {
+    bool need = false;
+    if (L.phase == PH_NEW) {
+      L.band += 1;
+      L.pass = 0;
+      need = L.band >= a.s.B;
+    }
+    const unsigned long long mask = __ballot(need);
+    if (mask) {
+      const int leader = __ffsll((long long)mask) - 1;
+      uint32_t base = 0;
+      if (lane == leader) base = atomicAdd(a.queue, (uint32_t)__popcll(mask));
+      base = __shfl(base, leader);
+      if (need) {
+        uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+        uint32_t item = base + rank;
+        if (item >= a.n_items) {
+          L.phase = PH_DONE;
+        } else {
+          item_to_pixel(a, item, L.pi, L.pj, L.blk);
+          L.band = 0;
+        }
+      }
+    }
+    if (L.phase == PH_NEW || L.phase == PH_REPLAY) start_path<ADJ>(L, c);
+    if (__ballot(L.phase != PH_DONE) == 0ull) break;
+    if (L.phase == PH_EXT || L.phase == PH_SHD) {
+      const bool ext = L.phase == PH_EXT;
+      f3 ro = ext ? L.pos : L.pos + L.sdir * L.soff;
+      f3 rd = ext ? L.dir : L.sdir;
+      HitD h = trace_closest(a.s, ro, rd, stk, stride, a.overflow);
+      if (ext) on_extension<ADJ>(L, c, h);
+      else on_shadow<ADJ>(L, c, h);
+    }
+  }
+  // per-wave counter reduction, one atomic per wave
+  unsigned long long seg = L.n_seg, trav = L.n_trav, nf = L.n_nf;
+  for (int off = 32; off >= 1; off >>= 1) {
+    seg += __shfl_xor(seg, off);
+    trav += __shfl_xor(trav, off);
+    nf += __shfl_xor(nf, off);
+  }
+  if (lane == 0) {
+    atomicAdd(a.counters + 0, seg);
+    atomicAdd(a.counters + 1, trav);
+    atomicAdd(a.counters + 2, nf);
+  }
+}
+
+__global__ __launch_bounds__(256) void sr_path_kernel(PathArgs a) {
+  __shared__ int lds_stack[SR_STACK_SIZE * 256];
+  path_loop<false>(a, lds_stack + threadIdx.x, nullptr);
+}
+
+__global__ __launch_bounds__(256) void sr_adjoint_kernel(PathArgs a) {
+  __shared__ int lds_stack[SR_STACK_SIZE * 256];
+  __shared__ float lds_grad[SR_MAX_PARAMS];
+  for (int i = threadIdx.x; i < a.adj.n_params; i += blockDim.x) lds_grad[i] = 0.0f;
+  __syncthreads();
+  path_loop<true>(a, lds_stack + threadIdx.x, lds_grad);
+  __syncthreads();
+  for (int i = threadIdx.x; i < a.adj.n_params; i += blockDim.x) {
+    float v = lds_grad[i];
+    if (v != 0.0f) atomicAdd(a.adj.grad + i, v * a.adj.scale);
+  }
+}
+
+static int grid_for(const void* fn, int* grid_out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return -1;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return -1;
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0) != hipSuccess) return -1;
+  if (per_cu < 1) per_cu = 1;
+  *grid_out = per_cu * prop.multiProcessorCount;
+  return 0;
+}
+
+int sr_path_grid(int* grid_out) { return grid_for((const void*)sr_path_kernel, grid_out); }
+int sr_adjoint_grid(int* grid_out) { return grid_for((const void*)sr_adjoint_kernel, grid_out); }
+
+int sr_launch_path(const PathArgs& a, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(sr_path_kernel, dim3(grid), dim3(256), 0, st, a);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int sr_launch_adjoint(const PathArgs& a, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(sr_adjoint_kernel, dim3(grid), dim3(256), 0, st, a);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}

@@ -37,6 +37,9 @@ SIGNATURES = {
     "sr_trace_closest": ([_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, ctypes.c_int], ctypes.c_int),
     "sr_render": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int, ctypes.c_int,
                    ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
+    "sr_render_adjoint": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int,
+                           ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                           ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
     "sr_resolve": ([_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                     ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
     "sr_stats": ([_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
