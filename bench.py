@@ -64,6 +64,142 @@ def cpu_baseline(flat, cfg, target_s=15.0):
                       f"C++ restatement of the reference semantics, not Taichi"}
 
 
+def adjoint_ratio(reps=2):
+    """adjoint : forward device-time ratio on the inverse-render workload
+    (volume/fluorescence scene, 256x256, 64 spp, depth 12, 126 parameters):
+    path kernel time of sr_render_adjoint over that of sr_render for the
+    same blocks and seed."""
+    import torch
+
+    from spectral import configs
+    from spectral.autograd import DifferentiableScene, inverse_volume_spec
+    from spectral.scene import flatten_scene
+
+    cfg = configs.inverse_volume()
+    flat = flatten_scene(cfg.scene, cfg.bands, cfg.wl_min, cfg.wl_max)
+    ds = DifferentiableScene(flat, inverse_volume_spec(), cfg.width, cfg.height, cfg.spp, cfg.depth, cfg.seed,
+                             device="cuda:0")
+    d_acc = torch.rand((cfg.width, cfg.height, cfg.bands), device="cuda:0")
+    fwd, adj = [], []
+    for _ in range(reps + 1):
+        ds.render()
+        fwd.append(ds.r.last_kernel_ms())
+        segs = ds.r.stats()["segments"]
+        ds.adjoint(d_acc)
+        adj.append(ds.r.last_kernel_ms())
+    fwd, adj = min(fwd[1:]), min(adj[1:])
+    return {"value": round(adj / fwd, 3), "forward_ms": round(fwd, 2), "adjoint_ms": round(adj, 2),
+            "workload": f"{cfg.name}: {flat.n_tris} tris, {cfg.bands} bands, {cfg.width}x{cfg.height}, "
+                        f"{cfg.spp} spp, depth {cfg.depth}, {ds.n_params} params ({segs} segments per forward)"}
+
+
+def bench_batch(spp_scale=8, graph=True, log=print):
+    """64-scene batch in one persistent launch (hipGraph) against sequential
+    per-scene launches of the same jobs.  spp is divided by spp_scale to
+    bound the run time (full spp: ~19G paths)."""
+    import numpy as np
+    import torch
+
+    from spectral import configs
+    from spectral.batch import Batch, tune
+    from spectral.render import Renderer
+    from spectral.scene import flatten_scene
+
+    t0 = time.time()
+    cfgs = configs.batch64()
+    rs = []
+    for c in cfgs:
+        f = flatten_scene(c.scene, c.bands, c.wl_min, c.wl_max)
+        rs.append((c, f.n_tris, Renderer(f, c.bands, c.wl_min, c.wl_max, width=c.width, height=c.height,
+                                         device="cuda:0")))
+    log(f"# batch64: {len(rs)} scenes, {sum(n for _, n, _ in rs)} tris, uploaded in {time.time() - t0:.1f} s")
+    spp = [max(1, c.spp // spp_scale) for c, _, _ in rs]
+    # autotune on one block per scene
+    best, timings = tune([(r, 1, c.depth, c.seed) for c, _, r in rs], blocks=(128, 256), grid_pcts=(100,),
+                         chunks=(64, 256, 1024), graph=graph, reps=1)
+    log(f"# autotuner: best (block, grid%, chunk) = {best}; " +
+        ", ".join(f"{k}: {v:.1f} ms" for k, v in timings.items()))
+    # batched: one launch over all scenes
+    for _, _, r in rs:
+        r.clear_pixel_space()
+    b = Batch()
+    b.set_config(*best)
+    for (c, _, r), n in zip(rs, spp):
+        b.submit(r, n, c.depth, c.seed)
+    b.run(graph)  # capture + warm
+    for _, _, r in rs:
+        r.clear_pixel_space()
+    out = b.run(graph)
+    segs = int(np.sum(out["segments"]))
+    batched = {"ms": round(out["total_ms"], 2), "Mseg_per_s": round(segs / out["total_ms"] / 1e3, 1),
+               "p50_latency_ms": round(float(np.percentile(out["latency_ms"], 50)), 2),
+               "p99_latency_ms": round(float(np.percentile(out["latency_ms"], 99)), 2)}
+    # sequential: one sr_render launch per scene, back to back, same order
+    lat = []
+    seq_segs = 0
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    start.record()
+    evs = []
+    for (c, _, r), n in zip(rs, spp):
+        r.clear_pixel_space()
+        r.render(n_blocks=n, max_depth=c.depth, seed=c.seed)
+        e = torch.cuda.Event(enable_timing=True)
+        e.record()
+        evs.append(e)
+    torch.cuda.synchronize()
+    lat = [start.elapsed_time(e) for e in evs]
+    for _, _, r in rs:
+        pass
+    seq_ms = lat[-1]
+    sequential = {"ms": round(seq_ms, 2), "Mseg_per_s": round(segs / seq_ms / 1e3, 1),
+                  "p50_latency_ms": round(float(np.percentile(lat, 50)), 2),
+                  "p99_latency_ms": round(float(np.percentile(lat, 99)), 2)}
+    return {"metric": "batch64 aggregate Mpath-segments/s and per-scene completion latency",
+            "workload": f"64 scenes (seed 2), spp/{spp_scale}, 31 bands, depth 12, "
+                        f"{sum(c.width * c.height * n * c.bands for (c, _, _), n in zip(rs, spp))} paths",
+            "segments": segs, "tuned_config": {"block": best[0], "grid_pct": best[1], "chunk": best[2]},
+            "graph": graph, "batched": batched, "sequential": sequential}
+
+
+def bench_trace(n=1 << 22, reps=5, log=print):
+    """Closest-hit traversal alone (sr_trace_closest, device pointers) on the
+    heightfield_1m scene: coherent pixel-centre primary rays and incoherent
+    rays (uniform origins in the box, uniform directions)."""
+    import numpy as np
+    import torch
+
+    from spectral import configs
+    from spectral.render import Renderer
+    from spectral.scene import flatten_scene
+
+    cfg = configs.heightfield_1m()
+    flat = flatten_scene(cfg.scene, cfg.bands, cfg.wl_min, cfg.wl_max)
+    r = Renderer(flat, cfg.bands, cfg.wl_min, cfg.wl_max, device="cuda:0")
+    rng = np.random.default_rng(0)
+    o_p, d_p = r.primary_rays(2048, 2048)
+    o_r = rng.uniform(-0.49, 0.49, (n, 3)).astype(np.float32)
+    d_r = rng.normal(size=(n, 3)).astype(np.float32)
+    d_r /= np.linalg.norm(d_r, axis=1, keepdims=True)
+    out = {}
+    for name, (o, d) in {"primary": (o_p, d_p), "random": (o_r, d_r)}.items():
+        og = torch.from_numpy(np.ascontiguousarray(o)).cuda()
+        dg = torch.from_numpy(np.ascontiguousarray(d)).cuda()
+        r.trace(og, dg)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            t, p, _ = r.trace(og, dg)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[name] = {"rays": int(og.shape[0]), "ms": round(ms, 3), "Grays_per_s": round(og.shape[0] / ms / 1e6, 3),
+                     "hit_fraction": round(float((p >= 0).float().mean()), 4)}
+    return {"metric": "closest-hit traversal rate, heightfield_1m (1,048,588 tris)", **out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +208,8 @@ def main():
     ap.add_argument("--workload", default="heightfield_1m")
     ap.add_argument("--res", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--spp-scale", type=int, default=8, help="batch64: divide each scene's spp by this")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration)")
@@ -86,6 +224,13 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_mib": args.calibrate_copy, "bytes_read": n * 4, "bytes_written": n * 4}))
+        return
+    if args.workload == "trace":
+        print(json.dumps(bench_trace()), flush=True)
+        return
+    if args.workload == "batch64":
+        print(json.dumps(bench_batch(args.spp_scale, log=lambda m: print(m, file=sys.stderr, flush=True))),
+              flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (multi-GPU sharding is design-only)",
@@ -163,6 +308,8 @@ def main():
         "nonfinite": nonfinite,
         "roofline": roof,
     }
+    if not args.no_adjoint:
+        out["adjoint_forward_ratio"] = adjoint_ratio()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(flat, cfg, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -41,6 +41,14 @@ typedef struct sr_handle sr_handle;
 /* sr_trace_closest flags */
 #define SR_DEVICE_PTRS 1
 
+/* sr_render_adjoint flags: leave out one family of gradient terms (for
+ * diagnosing estimator variance; 0 computes the full gradient). */
+#define SR_ADJ_SKIP_HG_SCORE 1     /* d log p / d g of HG direction sampling */
+#define SR_ADJ_SKIP_G_NEE 2        /* d / d g of the phase function in NEE */
+#define SR_ADJ_SKIP_SIG_SCORE 4    /* free-flight and absorb/scatter choice scores */
+#define SR_ADJ_SKIP_SIG_FACTOR 8   /* sigma_a in estimators and shadow transmittance */
+#define SR_ADJ_SKIP_ROUGH_SCORE 16 /* GGX half-vector sampling score */
+
 /* Last error message of this thread ("" when none). */
 const char* sr_last_error(void);
 
@@ -117,10 +125,11 @@ int sr_render(sr_handle* h, int width, int height, uint64_t seed, uint32_t strea
  *   roughness of material rough_mat (-1: none) at param index rough_idx;
  *   anistropy_factor g of material g_mat at g_idx;
  *   sigma_a = 4 pi k / lambda per band of material sig_mat at sig_off + band.
- * n_params <= 1024. */
+ * n_params <= 1024.  flags: SR_ADJ_SKIP_* mask, 0 for the full gradient. */
 int sr_render_adjoint(sr_handle* h, int width, int height, uint64_t seed, uint32_t stream, int first_block,
                       int n_blocks, int max_depth, const float* dev_d_accum, const int* alb_off, int rough_mat,
-                      int rough_idx, int g_mat, int g_idx, int sig_mat, int sig_off, int n_params, float* dev_grad);
+                      int rough_idx, int g_mat, int g_idx, int sig_mat, int sig_off, int n_params, int flags,
+                      float* dev_grad);
 
 /* color_rays (main.py:1609-1629): dev_accum [W][H][B] -> dev_rgb
  * [out_w][out_h][3] with nearest upsampling.  cmf [B][3] (host),
@@ -139,6 +148,32 @@ int sr_last_kernel_ms(sr_handle* h, float* ms);
 
 /* Blocks until all work of h is done. */
 int sr_sync(sr_handle* h);
+
+/* ---- Batched multi-scene launch (no reference counterpart: the reference
+ * renders one scene held in global fields, main.py:361-394, 1683-1709, with
+ * one kernel launch per band, main.py:1831).  A batch holds jobs -- each an
+ * sr_render call on its own scene handle -- and runs them all in ONE
+ * persistent launch fed from per-job device work queues, optionally
+ * captured once as a hipGraph and replayed. */
+typedef struct sr_batch sr_batch;
+
+#define SR_BATCH_GRAPH 1 /* sr_batch_launch: capture on first launch, replay after */
+
+int sr_batch_create(sr_batch** out);
+int sr_batch_destroy(sr_batch* b);
+/* Adds a job with sr_render's arguments.  A handle may appear once. */
+int sr_batch_submit(sr_batch* b, sr_handle* h, int width, int height, uint64_t seed, uint32_t stream,
+                    int first_block, int n_blocks, int max_depth, float* dev_accum);
+int sr_batch_clear(sr_batch* b);
+/* Launch shape: workgroup size (64/128/256), grid as a percentage of the
+ * occupancy-limited resident grid, items claimed per queue atomic. */
+int sr_batch_config(sr_batch* b, int block, int grid_pct, int chunk);
+/* Enqueues the batch on the batch's own stream (asynchronous). */
+int sr_batch_launch(sr_batch* b, int flags);
+/* Blocks; total device time, and per job (arrays of length = jobs, may be
+ * NULL): completion latency from the launch's first wave (device wall
+ * clock), path segments and traversals. */
+int sr_batch_wait(sr_batch* b, float* total_ms, float* job_ms, uint64_t* job_segments, uint64_t* job_traversals);
 
 #ifdef __cplusplus
 }

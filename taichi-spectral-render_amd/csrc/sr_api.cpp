@@ -87,7 +87,8 @@ struct sr_handle {
   std::vector<int> h_light_count;
   float cam_o[3], cam_u[3], cam_v[3], cam_w[3];
   // work
-  DevBuf sum, counters, queue, overflow, tmp, alb_off;
+  DevBuf sum, work, jobs, overflow, tmp, alb_off;
+  Job job_host;  // staging copy of the one-job table (kept alive for the async copy)
   int grid = 0, adj_grid = 0;
 };
 
@@ -163,6 +164,55 @@ static int upload_materials(sr_handle* h, const float* mat_f32, const int* mat_i
   return 0;
 }
 
+struct sr_batch {
+  int device = 0;
+  hipStream_t st = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  struct Spec {
+    sr_handle* h;
+    int W, H, first, nb, depth;
+    uint64_t seed;
+    uint32_t stream;
+    float* accum;
+  };
+  std::vector<Spec> specs;
+  std::vector<Job> host_jobs;
+  DevBuf jobs, work, overflow;
+  int block = 256, grid_pct = 100;
+  uint32_t chunk = 64;
+  bool dirty = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool launched = false;
+  double clock_khz = 100000.0;
+};
+
+// Work area of a launch over n jobs: queues[n], done[n] (u32), then
+// clocks[n + 1] and counters[4 n] (u64, 8-byte aligned).
+static size_t work_bytes(int n) { return ((size_t)8 * n + 7) / 8 * 8 + 8 * (size_t)(n + 1) + 32 * (size_t)n; }
+struct WorkView {
+  uint32_t* queues;
+  uint32_t* done;
+  unsigned long long* clocks;
+  unsigned long long* counters;
+};
+static WorkView work_view(void* base, int n) {
+  WorkView w;
+  w.queues = (uint32_t*)base;
+  w.done = w.queues + n;
+  w.clocks = (unsigned long long*)((char*)base + ((size_t)8 * n + 7) / 8 * 8);
+  w.counters = w.clocks + (n + 1);
+  return w;
+}
+// Graph-safe reset of the work area (kernels only, no memset nodes).
+static int reset_work(void* base, int n, hipStream_t st) {
+  WorkView w = work_view(base, n);
+  if (sr_launch_fill_u32(w.queues, (size_t)2 * n, 0u, st)) return -1;
+  if (sr_launch_fill_u32((uint32_t*)w.clocks, (size_t)2 * (n + 1), 0xFFFFFFFFu, st)) return -1;
+  if (sr_launch_fill_u32((uint32_t*)w.counters, (size_t)8 * n, 0u, st)) return -1;
+  return 0;
+}
+
 extern "C" {
 
 const char* sr_last_error(void) { return g_err.c_str(); }
@@ -193,14 +243,14 @@ int sr_create(int bands, float wl_min, float wl_max, sr_handle** out) {
     delete h;
     return fail(SR_EHIP, "occupancy query failed");
   }
-  int rc = h->counters.ensure(4 * sizeof(unsigned long long));
-  if (!rc) rc = h->queue.ensure(64);
+  int rc = h->work.ensure(work_bytes(1));
+  if (!rc) rc = h->jobs.ensure(sizeof(Job));
   if (!rc) rc = h->overflow.ensure(64);
   if (rc) {
     delete h;
     return rc;
   }
-  (void)hipMemset(h->counters.p, 0, h->counters.bytes);
+  (void)hipMemset(h->work.p, 0, h->work.bytes);
   (void)hipMemset(h->overflow.p, 0, h->overflow.bytes);
   *out = h;
   return 0;
@@ -211,7 +261,7 @@ int sr_destroy(sr_handle* h) {
   (void)hipSetDevice(h->device);
   if (h->st) (void)hipStreamSynchronize(h->st);
   DevBuf* bufs[] = {&h->nodes, &h->tris, &h->shade, &h->mats, &h->mat_eem, &h->eem, &h->lights, &h->lverts,
-                    &h->light_count, &h->light_area, &h->lam_m, &h->lam_nm, &h->sum, &h->counters, &h->queue,
+                    &h->light_count, &h->light_area, &h->lam_m, &h->lam_nm, &h->sum, &h->work, &h->jobs,
                     &h->overflow, &h->tmp, &h->alb_off};
   for (DevBuf* b : bufs) b->release();
   if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -454,35 +504,53 @@ static int check_render_args(sr_handle* h, int width, int height, int first_bloc
   return 0;
 }
 
-static int prepare_path_args(sr_handle* h, PathArgs& a, int width, int height, uint64_t seed, uint32_t stream,
-                             int first_block, int n_blocks, int max_depth) {
-  hipStream_t st = h->st;
+static void fill_job(sr_handle* h, Job& j, int width, int height, uint64_t seed, uint32_t stream, int first_block,
+                     int n_blocks, int max_depth) {
+  std::memset(&j, 0, sizeof(j));
+  fill_scene_dev(h, j.s);
+  j.sum = (float*)h->sum.p;
+  j.n_items = (uint32_t)((unsigned long long)width * height * (unsigned long long)n_blocks);
+  j.W = width;
+  j.H = height;
+  j.depth = max_depth;
+  j.first_block = first_block;
+  j.n_blocks = n_blocks;
+  j.tiled = (width % 8 == 0 && height % 8 == 0) ? 1 : 0;
+  j.seed_lo = (uint32_t)seed;
+  j.seed_hi = (uint32_t)(seed >> 32);
+  j.stream = stream;
+}
+
+static int ensure_sum(sr_handle* h, int width, int height, hipStream_t st) {
   const size_t n = (size_t)width * height * h->B;
   if (h->sum.bytes < n * 4) {
     int rc = h->sum.ensure(n * 4);
     if (rc) return rc;
     if (sr_launch_fill((float*)h->sum.p, n, 0.0f, st)) return fail(SR_EHIP, "fill launch failed");
   }
-  HIPCHK(hipMemsetAsync(h->counters.p, 0, 4 * sizeof(unsigned long long), st));
-  HIPCHK(hipMemsetAsync(h->queue.p, 0, 4, st));
+  return 0;
+}
+
+static int prepare_path_args(sr_handle* h, PathArgs& a, int width, int height, uint64_t seed, uint32_t stream,
+                             int first_block, int n_blocks, int max_depth) {
+  hipStream_t st = h->st;
+  int rc = ensure_sum(h, width, height, st);
+  if (rc) return rc;
+  fill_job(h, h->job_host, width, height, seed, stream, first_block, n_blocks, max_depth);
+  HIPCHK(hipMemcpyAsync(h->jobs.p, &h->job_host, sizeof(Job), hipMemcpyHostToDevice, st));
+  if (reset_work(h->work.p, 1, st)) return fail(SR_EHIP, "reset launch failed");
   HIPCHK(hipMemsetAsync(h->overflow.p, 0, 4, st));
   std::memset(&a, 0, sizeof(a));
-  fill_scene_dev(h, a.s);
-  a.sum = (float*)h->sum.p;
-  a.counters = (unsigned long long*)h->counters.p;
-  a.queue = (uint32_t*)h->queue.p;
+  WorkView w = work_view(h->work.p, 1);
+  a.jobs = (const Job*)h->jobs.p;
+  a.n_jobs = 1;
+  a.queues = w.queues;
+  a.done_items = w.done;
+  a.clocks = w.clocks;
+  a.counters = w.counters;
   a.overflow = (uint32_t*)h->overflow.p;
-  a.n_items = (uint32_t)((unsigned long long)width * height * (unsigned long long)n_blocks);
-  a.W = width;
-  a.H = height;
-  a.depth = max_depth;
-  a.first_block = first_block;
-  a.n_blocks = n_blocks;
-  a.tiled = (width % 8 == 0 && height % 8 == 0) ? 1 : 0;
-  a.seed_lo = (uint32_t)seed;
-  a.seed_hi = (uint32_t)(seed >> 32);
-  a.stream = stream;
-  a.adj.rough_mat = a.adj.g_mat = a.adj.sig_mat = -1;
+  a.chunk = 64;
+  a.adj.rough_mat = a.adj.g_mat = a.adj.sig_mat = -2;
   a.adj.rough_idx = a.adj.g_idx = a.adj.sig_off = -1;
   return 0;
 }
@@ -500,10 +568,10 @@ int sr_render(sr_handle* h, int width, int height, uint64_t seed, uint32_t strea
   if ((rc = prepare_path_args(h, a, width, height, seed, stream, first_block, n_blocks, max_depth))) return rc;
   hipStream_t st = h->st;
   int grid = h->grid;
-  const unsigned long long waves_needed = ((unsigned long long)a.n_items + 255) / 256;
+  const unsigned long long waves_needed = ((unsigned long long)h->job_host.n_items + 255) / 256;
   if ((unsigned long long)grid > waves_needed) grid = (int)waves_needed;
   HIPCHK(hipEventRecord(h->ev0, st));
-  if (sr_launch_path(a, grid, st)) return fail(SR_EHIP, "path kernel launch failed");
+  if (sr_launch_path(a, grid, 256, st)) return fail(SR_EHIP, "path kernel launch failed");
   HIPCHK(hipEventRecord(h->ev1, st));
   h->timed = true;
   const size_t n = (size_t)width * height * h->B;
@@ -514,7 +582,8 @@ int sr_render(sr_handle* h, int width, int height, uint64_t seed, uint32_t strea
 
 int sr_render_adjoint(sr_handle* h, int width, int height, uint64_t seed, uint32_t stream, int first_block,
                       int n_blocks, int max_depth, const float* dev_d_accum, const int* alb_off, int rough_mat,
-                      int rough_idx, int g_mat, int g_idx, int sig_mat, int sig_off, int n_params, float* dev_grad) {
+                      int rough_idx, int g_mat, int g_idx, int sig_mat, int sig_off, int n_params, int flags,
+                      float* dev_grad) {
   g_err.clear();
   int rc = check_render_args(h, width, height, first_block, n_blocks, max_depth, stream);
   if (rc) return rc;
@@ -543,13 +612,14 @@ int sr_render_adjoint(sr_handle* h, int width, int height, uint64_t seed, uint32
   a.adj.sig_mat = sig_mat < 0 ? -2 : sig_mat;
   a.adj.sig_off = sig_off;
   a.adj.n_params = n_params;
+  a.adj.flags = flags;
   a.adj.scale = 1.0f / (float)(first_block + n_blocks);
   hipStream_t st = h->st;
   int grid = h->adj_grid;
-  const unsigned long long waves_needed = ((unsigned long long)a.n_items + 255) / 256;
+  const unsigned long long waves_needed = ((unsigned long long)h->job_host.n_items + 255) / 256;
   if ((unsigned long long)grid > waves_needed) grid = (int)waves_needed;
   HIPCHK(hipEventRecord(h->ev0, st));
-  if (sr_launch_adjoint(a, grid, st)) return fail(SR_EHIP, "adjoint kernel launch failed");
+  if (sr_launch_adjoint(a, grid, 256, st)) return fail(SR_EHIP, "adjoint kernel launch failed");
   HIPCHK(hipEventRecord(h->ev1, st));
   h->timed = true;
   HIPCHK(hipStreamSynchronize(st));  // alb_off staging buffer is reused by the next call
@@ -581,7 +651,7 @@ int sr_stats(sr_handle* h, uint64_t* segments, uint64_t* traversals, uint64_t* n
   HIPCHK(hipSetDevice(h->device));
   unsigned long long c[4] = {0, 0, 0, 0};
   uint32_t ovf = 0;
-  HIPCHK(hipMemcpyAsync(c, h->counters.p, sizeof(c), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipMemcpyAsync(c, work_view(h->work.p, 1).counters, sizeof(c), hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipMemcpyAsync(&ovf, h->overflow.p, 4, hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
   if (segments) *segments = c[0];
@@ -602,6 +672,190 @@ int sr_last_kernel_ms(sr_handle* h, float* ms) {
 int sr_sync(sr_handle* h) {
   if (!h) return fail(SR_EINVAL, "null handle");
   HIPCHK(hipStreamSynchronize(h->st));
+  return 0;
+}
+
+
+// ---------------------------------------------------------------- batches
+static void batch_drop_graph(sr_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+int sr_batch_create(sr_batch** out) {
+  g_err.clear();
+  if (!out) return fail(SR_EINVAL, "out is NULL");
+  sr_batch* b = new sr_batch();
+  hipError_t e = hipGetDevice(&b->device);
+  if (e == hipSuccess) e = hipStreamCreate(&b->st);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+  int khz = 0;
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->device);
+  if (e != hipSuccess) {
+    delete b;
+    return fail(SR_EHIP, "HIP init failed: %s", hipGetErrorString(e));
+  }
+  if (khz > 0) b->clock_khz = (double)khz;
+  int rc = b->overflow.ensure(64);
+  if (rc) {
+    delete b;
+    return rc;
+  }
+  (void)hipMemset(b->overflow.p, 0, 64);
+  *out = b;
+  return 0;
+}
+
+int sr_batch_destroy(sr_batch* b) {
+  if (!b) return 0;
+  (void)hipSetDevice(b->device);
+  if (b->st) (void)hipStreamSynchronize(b->st);
+  batch_drop_graph(b);
+  b->jobs.release();
+  b->work.release();
+  b->overflow.release();
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->st) (void)hipStreamDestroy(b->st);
+  delete b;
+  return 0;
+}
+
+int sr_batch_submit(sr_batch* b, sr_handle* h, int width, int height, uint64_t seed, uint32_t stream,
+                    int first_block, int n_blocks, int max_depth, float* dev_accum) {
+  g_err.clear();
+  if (!b) return fail(SR_EINVAL, "null batch");
+  int rc = check_render_args(h, width, height, first_block, n_blocks, max_depth, stream);
+  if (rc) return rc;
+  if (!dev_accum) return fail(SR_EINVAL, "dev_accum is NULL");
+  if (h->device != b->device) return fail(SR_EINVAL, "scene handle is on another device");
+  for (const auto& sp : b->specs)
+    if (sp.h == h) return fail(SR_EINVAL, "a scene handle may appear once per batch (its block-sum buffer)");
+  b->specs.push_back({h, width, height, first_block, n_blocks, max_depth, seed, stream, dev_accum});
+  b->dirty = true;
+  return 0;
+}
+
+int sr_batch_clear(sr_batch* b) {
+  if (!b) return fail(SR_EINVAL, "null batch");
+  (void)hipStreamSynchronize(b->st);
+  b->specs.clear();
+  b->dirty = true;
+  batch_drop_graph(b);
+  return 0;
+}
+
+int sr_batch_config(sr_batch* b, int block, int grid_pct, int chunk) {
+  if (!b) return fail(SR_EINVAL, "null batch");
+  if (block != 64 && block != 128 && block != 256) return fail(SR_EINVAL, "block must be 64, 128 or 256");
+  if (grid_pct < 10 || grid_pct > 400) return fail(SR_EINVAL, "grid_pct outside [10, 400]");
+  if (chunk < 1 || chunk > (1 << 20)) return fail(SR_EINVAL, "bad chunk");
+  b->block = block;
+  b->grid_pct = grid_pct;
+  b->chunk = (uint32_t)chunk;
+  b->dirty = true;  // the captured graph bakes the launch shape in
+  return 0;
+}
+
+static int batch_enqueue(sr_batch* b, int grid) {
+  const int n = (int)b->specs.size();
+  hipStream_t st = b->st;
+  if (reset_work(b->work.p, n, st)) return fail(SR_EHIP, "reset launch failed");
+  PathArgs a;
+  std::memset(&a, 0, sizeof(a));
+  WorkView w = work_view(b->work.p, n);
+  a.jobs = (const Job*)b->jobs.p;
+  a.n_jobs = n;
+  a.queues = w.queues;
+  a.done_items = w.done;
+  a.clocks = w.clocks;
+  a.counters = w.counters;
+  a.overflow = (uint32_t*)b->overflow.p;
+  a.chunk = b->chunk;
+  a.adj.rough_mat = a.adj.g_mat = a.adj.sig_mat = -2;
+  a.adj.rough_idx = a.adj.g_idx = a.adj.sig_off = -1;
+  if (sr_launch_path(a, grid, b->block, st)) return fail(SR_EHIP, "path kernel launch failed");
+  for (int j = 0; j < n; ++j) {
+    const auto& sp = b->specs[j];
+    const size_t cnt = (size_t)sp.W * sp.H * sp.h->B;
+    if (sr_launch_finalize(sp.accum, b->host_jobs[j].sum, cnt, sp.first, sp.nb, st))
+      return fail(SR_EHIP, "finalize launch failed");
+  }
+  return 0;
+}
+
+int sr_batch_launch(sr_batch* b, int flags) {
+  g_err.clear();
+  if (!b) return fail(SR_EINVAL, "null batch");
+  if (b->specs.empty()) return fail(SR_ESTATE, "empty batch");
+  HIPCHK(hipSetDevice(b->device));
+  const int n = (int)b->specs.size();
+  if (b->dirty) {
+    HIPCHK(hipStreamSynchronize(b->st));
+    batch_drop_graph(b);
+    b->host_jobs.resize(n);
+    for (int j = 0; j < n; ++j) {
+      const auto& sp = b->specs[j];
+      int rc = ensure_sum(sp.h, sp.W, sp.H, b->st);
+      if (rc) return rc;
+      fill_job(sp.h, b->host_jobs[j], sp.W, sp.H, sp.seed, sp.stream, sp.first, sp.nb, sp.depth);
+    }
+    int rc = b->jobs.upload(b->host_jobs.data(), (size_t)n, b->st);
+    if (!rc) rc = b->work.ensure(work_bytes(n));
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(b->st));
+    b->dirty = false;
+  }
+  int per = 0;
+  if (sr_path_grid_bs(b->block, &per) != 0) return fail(SR_EHIP, "occupancy query failed");
+  int grid = std::max(1, per * b->grid_pct / 100);
+  HIPCHK(hipEventRecord(b->ev0, b->st));
+  if (flags & SR_BATCH_GRAPH) {
+    if (!b->exec) {
+      HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+      int rc = batch_enqueue(b, grid);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(b->st, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return fail(SR_EHIP, "graph capture failed: %s", hipGetErrorString(e));
+      b->graph = g;
+      HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    HIPCHK(hipGraphLaunch(b->exec, b->st));
+  } else {
+    int rc = batch_enqueue(b, grid);
+    if (rc) return rc;
+  }
+  HIPCHK(hipEventRecord(b->ev1, b->st));
+  b->launched = true;
+  return 0;
+}
+
+int sr_batch_wait(sr_batch* b, float* total_ms, float* job_ms, uint64_t* job_segments, uint64_t* job_traversals) {
+  if (!b) return fail(SR_EINVAL, "null batch");
+  if (!b->launched) return fail(SR_ESTATE, "nothing launched");
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipEventSynchronize(b->ev1));
+  const int n = (int)b->specs.size();
+  if (total_ms) HIPCHK(hipEventElapsedTime(total_ms, b->ev0, b->ev1));
+  std::vector<unsigned long long> clk((size_t)n + 1), cnt((size_t)4 * n);
+  WorkView w = work_view(b->work.p, n);
+  uint32_t ovf = 0;
+  HIPCHK(hipMemcpy(clk.data(), w.clocks, sizeof(unsigned long long) * (n + 1), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(cnt.data(), w.counters, sizeof(unsigned long long) * 4 * n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(&ovf, b->overflow.p, 4, hipMemcpyDeviceToHost));
+  for (int j = 0; j < n; ++j) {
+    if (job_ms) job_ms[j] = (float)((double)(clk[1 + j] - clk[0]) / b->clock_khz);
+    if (job_segments) job_segments[j] = cnt[4 * j];
+    if (job_traversals) job_traversals[j] = cnt[4 * j + 1];
+  }
+  if (ovf) return fail(SR_EOVERFLOW, "traversal stack overflow in the batch");
   return 0;
 }
 

@@ -131,10 +131,82 @@ __device__ __forceinline__ void intersect_tri(f3 o, f3 d, float4 a, float4 b, fl
   if (ok) { best_t = t; best_prim = prim; bu = u; bv = v; }
 }
 
+// Tests the 4 children of node `cur` against the ray; returns the nearest
+// hit child in `next` (SR_EMPTY when none) and pushes the others far-to-near.
+__device__ __forceinline__ void visit_node(const Node4* __restrict__ nodes, int cur, f3 inv, f3 oi, bool nx, bool ny,
+                                           bool nz, float tmax_, int* stk, int stride, int& sp, int& next,
+                                           uint32_t* overflow) {
+  const float4* np = reinterpret_cast<const float4*>(nodes + cur);
+  float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+  int4 ch = *reinterpret_cast<const int4*>(np + 6);
+  float4 nxv = nx ? hx : lx, fxv = nx ? lx : hx;
+  float4 nyv = ny ? hy : ly, fyv = ny ? ly : hy;
+  float4 nzv = nz ? hz : lz, fzv = nz ? lz : hz;
+  float tn[4], tf[4];
+  const float* pnx = &nxv.x; const float* pfx = &fxv.x;
+  const float* pny = &nyv.x; const float* pfy = &fyv.x;
+  const float* pnz = &nzv.x; const float* pfz = &fzv.x;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    float a0 = fmaf(pnx[c], inv.x, -oi.x), a1 = fmaf(pny[c], inv.y, -oi.y), a2 = fmaf(pnz[c], inv.z, -oi.z);
+    float b0 = fmaf(pfx[c], inv.x, -oi.x), b1 = fmaf(pfy[c], inv.y, -oi.y), b2 = fmaf(pfz[c], inv.z, -oi.z);
+    tn[c] = fmaxf(fmaxf(a0, a1), fmaxf(a2, 0.0f));
+    tf[c] = fminf(fminf(b0, b1), fminf(b2, tmax_));
+  }
+  int cref[4] = {ch.x, ch.y, ch.z, ch.w};
+  float key[4];
+  int hitc = 0;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    // conservative slack for the fma-rounded slab distances
+    bool hit = tn[c] <= tf[c] * 1.0000005f + 1e-7f && cref[c] != SR_EMPTY;
+    key[c] = hit ? tn[c] : INFINITY;
+    hitc += hit ? 1 : 0;
+  }
+  if (hitc == 0) {
+    next = SR_EMPTY;
+    return;
+  }
+#define SR_CSWAP(i, j)                                                   \
+  if (key[j] < key[i]) {                                                 \
+    float tk = key[i]; key[i] = key[j]; key[j] = tk;                     \
+    int tr = cref[i]; cref[i] = cref[j]; cref[j] = tr;                   \
+  }
+  SR_CSWAP(0, 1) SR_CSWAP(2, 3) SR_CSWAP(0, 2) SR_CSWAP(1, 3) SR_CSWAP(1, 2)
+#undef SR_CSWAP
+  next = cref[0];
+#pragma unroll
+  for (int c = 3; c >= 1; --c) {
+    if (c < hitc) {
+      if (sp < SR_STACK_SIZE) stk[(sp++) * stride] = cref[c];
+      else atomicOr(overflow, 1u);
+    }
+  }
+}
+
+__device__ __forceinline__ void visit_leaf(const float4* __restrict__ tris, int ref, f3 o, f3 d, HitD& h) {
+  const uint32_t r = (uint32_t)ref;
+  const int first = (int)((r & ~SR_LEAF_BIT) >> 3);
+  const int cnt = (int)(r & 7u);
+  for (int k = 0; k < cnt; ++k) {
+    const float4* tp = tris + (size_t)(first + k) * 3;
+    intersect_tri(o, d, tp[0], tp[1], tp[2], h.t, h.prim, h.u, h.v);
+  }
+}
+
 // Closest hit over the 4-wide BVH.  `stk` is this lane's LDS stack column
 // (entry i at stk[i * stride]).  Children are visited nearest-first; boxes
 // beyond the current best t are culled (ties kept: <=).
-__device__ __forceinline__ HitD trace_closest(const SceneDev& s, f3 o, f3 d, int* stk, int stride, uint32_t* overflow) {
+//
+// Loop structure (while-while with speculative traversal): a lane walks inner
+// nodes until it meets a leaf, parks that leaf and keeps walking nodes until
+// every lane of the wave has a leaf (or is finished); then all lanes
+// intersect their leaves together.  This keeps the wave in one of the two
+// loop bodies instead of splitting it between nodes and leaves each step.
+// `nodes` / `tris` are passed by value so they sit in scalar registers for
+// the whole traversal (no reloads inside the loops).
+__device__ __forceinline__ HitD trace_closest(const Node4* __restrict__ nodes, const float4* __restrict__ tris, f3 o,
+                                              f3 d, int* stk, int stride, uint32_t* overflow) {
   HitD h;
   h.t = 1e9f;  // HitRecord(t=1e9), main.py:833
   h.prim = INT32_MIN;
@@ -147,68 +219,30 @@ __device__ __forceinline__ HitD trace_closest(const SceneDev& s, f3 o, f3 d, int
   f3 oi = mk(o.x * inv.x, o.y * inv.y, o.z * inv.z);
   const bool nx = inv.x < 0.f, ny = inv.y < 0.f, nz = inv.z < 0.f;
   int sp = 0;
-  int cur = 0;  // root node
-  while (true) {
-    if (cur >= 0) {
-      const float4* np = reinterpret_cast<const float4*>(s.nodes + cur);
-      float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-      int4 ch = *reinterpret_cast<const int4*>(np + 6);
-      float4 nxv = nx ? hx : lx, fxv = nx ? lx : hx;
-      float4 nyv = ny ? hy : ly, fyv = ny ? ly : hy;
-      float4 nzv = nz ? hz : lz, fzv = nz ? lz : hz;
-      float tn[4], tf[4];
-      const float* pnx = &nxv.x; const float* pfx = &fxv.x;
-      const float* pny = &nyv.x; const float* pfy = &fyv.x;
-      const float* pnz = &nzv.x; const float* pfz = &fzv.x;
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        float a0 = fmaf(pnx[c], inv.x, -oi.x), a1 = fmaf(pny[c], inv.y, -oi.y), a2 = fmaf(pnz[c], inv.z, -oi.z);
-        float b0 = fmaf(pfx[c], inv.x, -oi.x), b1 = fmaf(pfy[c], inv.y, -oi.y), b2 = fmaf(pfz[c], inv.z, -oi.z);
-        tn[c] = fmaxf(fmaxf(a0, a1), fmaxf(a2, 0.0f));
-        tf[c] = fminf(fminf(b0, b1), fminf(b2, h.t));
+  int cur = 0;       // root node; SR_EMPTY = nothing left
+  int leaf = SR_EMPTY;  // parked leaf
+  while (cur != SR_EMPTY || leaf != SR_EMPTY) {
+    // node loop
+    while (cur != SR_EMPTY && cur >= 0) {
+      int next;
+      visit_node(nodes, cur, inv, oi, nx, ny, nz, h.t, stk, stride, sp, next, overflow);
+      if (next == SR_EMPTY) next = sp > 0 ? stk[(--sp) * stride] : SR_EMPTY;
+      cur = next;
+      if (cur != SR_EMPTY && cur < 0 && leaf == SR_EMPTY) {  // park the first leaf, keep walking
+        leaf = cur;
+        cur = sp > 0 ? stk[(--sp) * stride] : SR_EMPTY;
       }
-      int cref[4] = {ch.x, ch.y, ch.z, ch.w};
-      // Conservative slack for the fma-rounded slab distances.
-      float key[4];
-      int hitc = 0;
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        bool hit = tn[c] <= tf[c] * 1.0000005f + 1e-7f && cref[c] != SR_EMPTY;
-        key[c] = hit ? tn[c] : INFINITY;
-        hitc += hit ? 1 : 0;
+      const bool waiting = cur != SR_EMPTY && cur >= 0 && leaf == SR_EMPTY;
+      if (__ballot(waiting) == 0ull) break;
+    }
+    // leaf loop
+    while (leaf != SR_EMPTY) {
+      visit_leaf(tris, leaf, o, d, h);
+      leaf = SR_EMPTY;
+      if (cur != SR_EMPTY && cur < 0) {
+        leaf = cur;
+        cur = sp > 0 ? stk[(--sp) * stride] : SR_EMPTY;
       }
-      if (hitc == 0) {
-        if (sp == 0) break;
-        cur = stk[(--sp) * stride];
-        continue;
-      }
-      // sort the 4 (key, ref) pairs ascending with a 5-comparator network
-#define SR_CSWAP(i, j)                                                   \
-  if (key[j] < key[i]) {                                                 \
-    float tk = key[i]; key[i] = key[j]; key[j] = tk;                     \
-    int tr = cref[i]; cref[i] = cref[j]; cref[j] = tr;                   \
-  }
-      SR_CSWAP(0, 1) SR_CSWAP(2, 3) SR_CSWAP(0, 2) SR_CSWAP(1, 3) SR_CSWAP(1, 2)
-#undef SR_CSWAP
-      cur = cref[0];
-      // push the others far-to-near so the nearer pops first
-#pragma unroll
-      for (int c = 3; c >= 1; --c) {
-        if (c < hitc) {
-          if (sp < SR_STACK_SIZE) stk[(sp++) * stride] = cref[c];
-          else atomicOr(overflow, 1u);
-        }
-      }
-    } else {
-      const uint32_t ref = (uint32_t)cur;
-      const int first = (int)((ref & ~SR_LEAF_BIT) >> 3);
-      const int cnt = (int)(ref & 7u) + 1;
-      for (int k = 0; k < cnt; ++k) {
-        const float4* tp = s.tris + (size_t)(first + k) * 3;
-        intersect_tri(o, d, tp[0], tp[1], tp[2], h.t, h.prim, h.u, h.v);
-      }
-      if (sp == 0) break;
-      cur = stk[(--sp) * stride];
     }
   }
   if (h.prim == INT32_MIN) h.prim = -1;

@@ -4,6 +4,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "../../include/spectral_render.h"
 #include "sr_device.h"
 
 #define SR_MAX_PARAMS 1024
@@ -17,19 +18,30 @@ struct AdjMap {
   float* grad;           // [n_params], accumulated into
   const int* alb_off;    // [M] device array
   int rough_mat, rough_idx, g_mat, g_idx, sig_mat, sig_off, n_params;
+  int flags;             // SR_ADJ_SKIP_* term mask (diagnostics); 0 = all terms
   float scale;           // 1 / (first_block + n_blocks)
 };
 
-struct PathArgs {
+// One scene render: the unit of work of the persistent path kernel.  A
+// single render is a table of one job; a batch is a table of many.
+struct Job {
   SceneDev s;
-  float* sum;                     // [W][H][B] block sums of this launch (zero on entry)
-  unsigned long long* counters;   // segments, traversals, non-finite drops
-  uint32_t* queue;                // next work item (zero on entry)
-  uint32_t* overflow;             // traversal stack overflow flag
-  uint32_t n_items;               // W * H * n_blocks
+  float* sum;        // [W][H][B] block sums of this launch (zero on entry)
+  uint32_t n_items;  // W * H * n_blocks
   int W, H, depth, first_block, n_blocks, tiled;
   uint32_t seed_lo, seed_hi, stream;
-  AdjMap adj;
+};
+
+struct PathArgs {
+  const Job* jobs;                // [n_jobs] device table
+  int n_jobs;
+  uint32_t* queues;               // [n_jobs] next work item (zero on entry)
+  uint32_t* done_items;           // [n_jobs] completed items (zero on entry)
+  unsigned long long* clocks;     // [n_jobs + 1] wall clock: [0] launch start (min), [1 + j] job j done
+  unsigned long long* counters;   // [n_jobs][4]: segments, traversals, non-finite drops
+  uint32_t* overflow;             // traversal stack overflow flag
+  uint32_t chunk;                 // items claimed per queue atomic (per wave)
+  AdjMap adj;                     // adjoint only (single job)
 };
 
 struct ResolveParams {
@@ -42,8 +54,11 @@ int sr_launch_trace(const SceneDev& s, int n, const float* o, const float* d, fl
                     uint32_t* overflow, hipStream_t st);
 int sr_path_grid(int* grid_out);
 int sr_adjoint_grid(int* grid_out);
-int sr_launch_path(const PathArgs& a, int grid, hipStream_t st);
-int sr_launch_adjoint(const PathArgs& a, int grid, hipStream_t st);
+int sr_path_grid_bs(int block, int* grid_out);
+size_t sr_path_lds_bytes(int block, bool adjoint);
+int sr_launch_path(const PathArgs& a, int grid, int block, hipStream_t st);
+int sr_launch_adjoint(const PathArgs& a, int grid, int block, hipStream_t st);
+int sr_launch_fill_u32(uint32_t* p, size_t n_words, uint32_t v, hipStream_t st);
 int sr_launch_finalize(float* mean, float* sum, size_t n, int first, int nb, hipStream_t st);
 int sr_launch_fill(float* p, size_t n, float v, hipStream_t st);
 int sr_launch_resolve(const float* acc, int W, int H, int B, const ResolveParams& p, float* out, int OW, int OH,

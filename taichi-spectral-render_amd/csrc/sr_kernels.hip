@@ -16,7 +16,7 @@ __global__ __launch_bounds__(256) void sr_trace_kernel(SceneDev s, int n, const 
   if (r >= n) return;
   f3 o = mk(orig[r * 3], orig[r * 3 + 1], orig[r * 3 + 2]);
   f3 d = mk(dir[r * 3], dir[r * 3 + 1], dir[r * 3 + 2]);
-  HitD h = trace_closest(s, o, d, lds_stack + threadIdx.x, blockDim.x, overflow);
+  HitD h = trace_closest(s.nodes, s.tris, o, d, lds_stack + threadIdx.x, blockDim.x, overflow);
   out_t[r] = h.t;
   out_prim[r] = h.prim;
   out_uv[r * 2] = h.prim >= 0 ? h.u : 0.0f;

@@ -18,8 +18,9 @@
 #define SR_MAX_BANDS 64
 
 // Child reference in Node4::child: >= 0 is an inner node index; with the
-// sign bit set it is a leaf: bits 3..30 = first TriRec, bits 0..2 = count-1.
-// An empty slot has child = SR_EMPTY and an inverted (never-hit) box.
+// sign bit set it is a leaf: bits 3..30 = first TriRec, bits 0..2 = count
+// (1..SR_LEAF_MAX, never 0, so no leaf equals SR_EMPTY).  An empty slot has
+// child = SR_EMPTY and an inverted (never-hit) box.
 #define SR_EMPTY ((int)0x80000000)
 #define SR_LEAF_BIT 0x80000000u
 
@@ -44,5 +45,5 @@ struct alignas(16) LightRec {
 };
 
 static inline int sr_leaf_ref(uint32_t first, uint32_t count) {
-  return (int)(SR_LEAF_BIT | (first << 3) | (count - 1u));
+  return (int)(SR_LEAF_BIT | (first << 3) | count);
 }

@@ -25,7 +25,7 @@
 #include "sr_dual.h"
 #include "sr_kernels.h"
 
-enum { PH_NEW = 0, PH_EXT = 1, PH_SHD = 2, PH_DONE = 3, PH_REPLAY = 4 };
+enum { PH_NEW = 0, PH_EXT = 1, PH_SHD = 2, PH_REPLAY = 3, PH_PARK = 4, PH_FIN = 5 };
 
 struct Lane {
   int phase, band, pi, pj, blk;
@@ -42,20 +42,24 @@ struct Lane {
   f3 sdir;
   float smax, strav, stot, soff;
   int sprim, svol, sstep;
+  float fdirect;  // NEE contribution awaiting finish_segment (PH_FIN)
   uint32_t n_seg, n_trav, n_nf;
   // adjoint only
   int pass;
-  float Lw, Eb, sp_dr, stot_dsig;
+  float Lw, Eb, sp_dr, stot_dsig, hg_sc;
 };
 
+// The lane's current job (wave-uniform) plus the adjoint configuration.
 template <bool ADJ>
 struct Ctx {
-  const PathArgs& a;
+  const Job& a;
+  const AdjMap& adj;
   float* lds_grad;
 };
 
-__device__ __forceinline__ float dacc(const PathArgs& a, const Lane& L, int band) {
-  return a.adj.d_accum[((size_t)L.pi * a.H + L.pj) * a.s.B + band];
+template <bool ADJ>
+__device__ __forceinline__ float dacc(const Ctx<ADJ>& c, const Lane& L, int band) {
+  return c.adj.d_accum[((size_t)L.pi * c.a.H + L.pj) * c.a.s.B + band];
 }
 
 template <bool ADJ>
@@ -82,7 +86,7 @@ __device__ __forceinline__ void add_contrib(Lane& L, const Ctx<ADJ>& c, float v)
   }
   if (v == 0.0f) return;
   if (ADJ) {
-    float w = dacc(c.a, L, L.active) * v;
+    float w = dacc(c, L, L.active) * v;
     if (L.pass == 0) L.Lw += w;
     else L.Eb += w;
     return;
@@ -108,7 +112,7 @@ __device__ __forceinline__ void end_path(Lane& L, const Ctx<ADJ>& c) {
 // Camera ray of cast_rays, main.py:1506-1529.
 template <bool ADJ>
 __device__ __forceinline__ void start_path(Lane& L, const Ctx<ADJ>& c) {
-  const PathArgs& a = c.a;
+  const Job& a = c.a;
   const SceneDev& s = a.s;
   L.rng.init(a.seed_lo, a.seed_hi, (uint32_t)(L.pi * a.H + L.pj), (uint32_t)L.blk,
              (uint32_t)L.band | (a.stream << 8));
@@ -207,7 +211,7 @@ __device__ __forceinline__ void evaluate_bsdf_d(const SceneDev& s, const AdjMap&
 // contribution (accumulated after the scatter), needed by the adjoint.
 template <bool ADJ>
 __device__ __forceinline__ void scatter_surface(Lane& L, const Ctx<ADJ>& c, float direct) {
-  const PathArgs& a = c.a;
+  const Job& a = c.a;
   const SceneDev& s = a.s;
   const int ab = L.active;
   MatRec sm = get_mat(s, L.emat, ab);
@@ -320,11 +324,11 @@ __device__ __forceinline__ void scatter_surface(Lane& L, const Ctx<ADJ>& c, floa
     g_NdotL = NdotL; g_NdotH = NdotH; g_NdotV = NdotV; g_c = cc;
   }
   if (ADJ && L.pass == 1) {
-    const AdjMap& m = a.adj;
+    const AdjMap& m = c.adj;
     const bool rough = L.emat == m.rough_mat;
     const int aoff = m.alb_off[L.emat];
     // radiance that follows this scatter (excludes this segment's NEE term)
-    float R = L.Lw - L.Eb - (isfinite(direct) ? dacc(a, L, ab) * direct : 0.0f);
+    float R = L.Lw - L.Eb - (isfinite(direct) ? dacc(c, L, ab) * direct : 0.0f);
     Dl r = dvar(sm.r, SL_ROUGH, rough);
     Dl est_d = dc(est);
     Dl pdf_d = dc(pdf);
@@ -349,7 +353,7 @@ __device__ __forceinline__ void scatter_surface(Lane& L, const Ctx<ADJ>& c, floa
     if (aoff >= 0) emit_grad(c, aoff + ab, R * dl[SL_ALB]);
     if (rough) {
       // half vector drawn from D_{a2 = r^2}(h) (n.h) regardless of the branch
-      float sc = ggx_sample_score_r(tmax(1e-4f, dot(N, H)), sm.r);
+      float sc = (m.flags & SR_ADJ_SKIP_ROUGH_SCORE) ? 0.0f : ggx_sample_score_r(tmax(1e-4f, dot(N, H)), sm.r);
       emit_grad(c, m.rough_idx, R * (dl[SL_ROUGH] + sc));
     }
     L.sp_dr = rough ? pdf_d.d[SL_ROUGH] : 0.0f;
@@ -365,6 +369,11 @@ __device__ __forceinline__ void scatter_surface(Lane& L, const Ctx<ADJ>& c, floa
 // End of one path-loop iteration (main.py:1565-1590).
 template <bool ADJ>
 __device__ __forceinline__ void finish_segment(Lane& L, const Ctx<ADJ>& c, float direct) {
+  if (ADJ && L.pass == 1 && L.hg_sc != 0.0f) {
+    float R = L.Lw - L.Eb - (isfinite(direct) ? dacc(c, L, L.active) * direct : 0.0f);
+    emit_grad(c, c.adj.g_idx, R * L.hg_sc);
+    L.hg_sc = 0.0f;
+  }
   if (L.eaction == 1) {
     L.pl += 1;
     scatter_surface<ADJ>(L, c, direct);
@@ -394,7 +403,7 @@ __device__ __forceinline__ void finish_segment(Lane& L, const Ctx<ADJ>& c, float
 // cast_rays event dispatch (main.py:1540-1578) up to the NEE shadow walk.
 template <bool ADJ>
 __device__ __forceinline__ void on_extension(Lane& L, const Ctx<ADJ>& c, const HitD& h) {
-  const PathArgs& a = c.a;
+  const Job& a = c.a;
   const SceneDev& s = a.s;
   L.n_seg++;
   L.n_trav++;
@@ -476,8 +485,9 @@ __device__ __forceinline__ void on_extension(Lane& L, const Ctx<ADJ>& c, const H
     end_path<ADJ>(L, c);  // escaped (main.py:1541-1543)
     return;
   }
+  if (ADJ) L.hg_sc = 0.0f;
   if (ADJ && L.pass == 1) {
-    const AdjMap& m = a.adj;
+    const AdjMap& m = c.adj;
     const float R = L.Lw - L.Eb;  // everything from this segment on carries est
     if (L.vol == m.sig_mat) {
       Dl sad = dvar(sa, SL_SIG, true);
@@ -499,11 +509,15 @@ __device__ __forceinline__ void on_extension(Lane& L, const Ctx<ADJ>& c, const H
         score = 1.0f / st - travel;  // free-flight density
         score += absorb_branch ? (sa > 0.f ? 1.0f / sa - 1.0f / st : 0.0f) : -1.0f / st;
       }
+      if (m.flags & SR_ADJ_SKIP_SIG_SCORE) score = 0.0f;
+      if (m.flags & SR_ADJ_SKIP_SIG_FACTOR) dl[SL_SIG] = 0.0f;
       emit_grad(c, m.sig_off + ab, R * (dl[SL_SIG] + score));
     }
-    if (L.eaction == 2 && L.vol == m.g_mat) {
-      emit_grad(c, m.g_idx, R * hg_score_g(dot(L.edir, L.dir), vm.g));
-    }
+    // the HG direction only affects what follows this segment's NEE term:
+    // its score is applied in finish_segment, once that term is known
+    L.hg_sc = (L.eaction == 2 && L.vol == m.g_mat && !(m.flags & SR_ADJ_SKIP_HG_SCORE))
+                  ? hg_score_g(dot(L.edir, L.dir), vm.g)
+                  : 0.0f;
   }
   L.pos = L.pos + L.dir * travel;
   L.tp *= est;
@@ -519,7 +533,7 @@ __device__ __forceinline__ void on_extension(Lane& L, const Ctx<ADJ>& c, const H
       float contrib = L.tp * smt.em * w;
       if (ADJ && L.pass == 1 && L.sp_dr != 0.0f && isfinite(contrib)) {
         float dlogw = 2.0f / L.sp - 2.0f * L.sp / den;
-        emit_grad(c, a.adj.rough_idx, dacc(a, L, L.active) * contrib * dlogw * L.sp_dr);
+        emit_grad(c, c.adj.rough_idx, dacc(c, L, L.active) * contrib * dlogw * L.sp_dr);
       }
       add_contrib<ADJ>(L, c, contrib);
       end_path<ADJ>(L, c);
@@ -531,7 +545,8 @@ __device__ __forceinline__ void on_extension(Lane& L, const Ctx<ADJ>& c, const H
   L.nee_surf = surf;
   const int nl = s.light_count[ab];
   if (nl <= 0) {
-    finish_segment<ADJ>(L, c, 0.0f);
+    L.fdirect = 0.0f;
+    L.phase = PH_FIN;
     return;
   }
   const LightRec* lr = s.lights + (size_t)ab * s.max_lights;
@@ -562,13 +577,13 @@ __device__ __forceinline__ void on_extension(Lane& L, const Ctx<ADJ>& c, const H
 // on the last step, the rest of sample_direct_light (main.py:1329-1356).
 template <bool ADJ>
 __device__ __forceinline__ void on_shadow(Lane& L, const Ctx<ADJ>& c, const HitD& h) {
-  const PathArgs& a = c.a;
+  const Job& a = c.a;
   const SceneDev& s = a.s;
   L.n_trav++;
   const int ab = L.active;
   MatRec vm = get_mat(s, L.svol, ab);
   float absorb = (4.0f * SR_PI * vm.k) / s.lam_m[ab];
-  const bool sig_span = ADJ && L.svol == a.adj.sig_mat;
+  const bool sig_span = ADJ && L.svol == c.adj.sig_mat;
   L.stot *= expf(-(absorb + vm.ss) * h.t);
   if (sig_span) L.stot_dsig -= h.t;
   bool done = false;
@@ -622,7 +637,7 @@ __device__ __forceinline__ void on_shadow(Lane& L, const Ctx<ADJ>& c, const HitD
     float w = (pdfl * pdfl) / (pdfl * pdfl + pdf * pdf + 1e-6f);
     direct = L.tp * emis * factor * w;
     if (ADJ && L.pass == 1 && isfinite(direct) && direct != 0.0f) {
-      const AdjMap& m = a.adj;
+      const AdjMap& m = c.adj;
       Dl bd, pd;
       if (L.nee_surf != -1) {
         evaluate_bsdf_d(s, m, L.vol, L.nee_surf, -L.dir, L.sdir, L.esn, ab, bd, pd);
@@ -636,18 +651,22 @@ __device__ __forceinline__ void on_shadow(Lane& L, const Ctx<ADJ>& c, const HitD
       dlog_into(wd, lw);
       // accumulated into the band after this segment's update (main.py:1586-1590)
       const int band_acc = L.eaction == 3 ? L.enew : ab;
-      const float dv = dacc(a, L, band_acc) * direct;
+      const float dv = dacc(c, L, band_acc) * direct;
       if (L.nee_surf != -1 && m.alb_off[L.nee_surf] >= 0)
         emit_grad(c, m.alb_off[L.nee_surf] + ab, dv * (lb[SL_ALB] + lw[SL_ALB]));
       if (L.nee_surf == m.rough_mat) emit_grad(c, m.rough_idx, dv * (lb[SL_ROUGH] + lw[SL_ROUGH]));
-      if (L.nee_surf == -1 && L.vol == m.g_mat) emit_grad(c, m.g_idx, dv * (lb[SL_G] + lw[SL_G]));
-      if (m.sig_mat >= 0) emit_grad(c, m.sig_off + ab, dv * L.stot_dsig);
+      if (L.nee_surf == -1 && L.vol == m.g_mat && !(m.flags & SR_ADJ_SKIP_G_NEE))
+        emit_grad(c, m.g_idx, dv * (lb[SL_G] + lw[SL_G]));
+      if (m.sig_mat >= 0 && !(m.flags & SR_ADJ_SKIP_SIG_FACTOR)) emit_grad(c, m.sig_off + ab, dv * L.stot_dsig);
     }
   }
-  finish_segment<ADJ>(L, c, direct);
+  // finish_segment runs once per loop iteration for every lane that reaches
+  // it (from either handler), so its large scatter_surface body is shared
+  L.fdirect = direct;
+  L.phase = PH_FIN;
 }
 
-__device__ __forceinline__ void item_to_pixel(const PathArgs& a, uint32_t item, int& pi, int& pj, int& blk) {
+__device__ __forceinline__ void item_to_pixel(const Job& a, uint32_t item, int& pi, int& pj, int& blk) {
   const uint32_t npix = (uint32_t)a.W * (uint32_t)a.H;
   blk = a.first_block + (int)(item / npix);
   uint32_t r = item % npix;
@@ -662,80 +681,135 @@ __device__ __forceinline__ void item_to_pixel(const PathArgs& a, uint32_t item, 
   }
 }
 
-template <bool ADJ>
-__device__ __forceinline__ void path_loop(const PathArgs& a, int* stk, float* lds_grad) {
-  const Ctx<ADJ> c{a, lds_grad};
-  const int stride = blockDim.x;
-  const int lane = (int)__lane_id();
-  Lane L;
-  L.phase = PH_NEW;
-  L.band = a.s.B;
-  L.pi = L.pj = L.blk = 0;
-  L.acc = 0.0f;
-  L.acc_band = 0;
-  L.n_seg = L.n_trav = L.n_nf = 0;
-  L.pass = 0;
-  L.Lw = L.Eb = L.sp_dr = L.stot_dsig = 0.0f;
-  while (true) {
-    bool need = false;
-    if (L.phase == PH_NEW) {
-      L.band += 1;
-      L.pass = 0;
-      need = L.band >= a.s.B;
-    }
-    const unsigned long long mask = __ballot(need);
-    if (mask) {
-      const int leader = __ffsll((long long)mask) - 1;
-      uint32_t base = 0;
-      if (lane == leader) base = atomicAdd(a.queue, (uint32_t)__popcll(mask));
-      base = __shfl(base, leader);
-      if (need) {
-        uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-        uint32_t item = base + rank;
-        if (item >= a.n_items) {
-          L.phase = PH_DONE;
-        } else {
-          item_to_pixel(a, item, L.pi, L.pj, L.blk);
-          L.band = 0;
-        }
-      }
-    }
-    if (L.phase == PH_NEW || L.phase == PH_REPLAY) start_path<ADJ>(L, c);
-    if (__ballot(L.phase != PH_DONE) == 0ull) break;
-    if (L.phase == PH_EXT || L.phase == PH_SHD) {
-      const bool ext = L.phase == PH_EXT;
-      f3 ro = ext ? L.pos : L.pos + L.sdir * L.soff;
-      f3 rd = ext ? L.dir : L.sdir;
-      HitD h = trace_closest(a.s, ro, rd, stk, stride, a.overflow);
-      if (ext) on_extension<ADJ>(L, c, h);
-      else on_shadow<ADJ>(L, c, h);
-    }
-  }
-  // per-wave counter reduction, one atomic per wave
+// Per-wave flush of the lane counters into job j's slot.
+__device__ __forceinline__ void flush_counters(const PathArgs& P, int j, Lane& L, int lane) {
   unsigned long long seg = L.n_seg, trav = L.n_trav, nf = L.n_nf;
   for (int off = 32; off >= 1; off >>= 1) {
     seg += __shfl_xor(seg, off);
     trav += __shfl_xor(trav, off);
     nf += __shfl_xor(nf, off);
   }
-  if (lane == 0) {
-    atomicAdd(a.counters + 0, seg);
-    atomicAdd(a.counters + 1, trav);
-    atomicAdd(a.counters + 2, nf);
+  if (lane == 0 && (seg | trav | nf)) {
+    atomicAdd(P.counters + 4 * j + 0, seg);
+    atomicAdd(P.counters + 4 * j + 1, trav);
+    atomicAdd(P.counters + 4 * j + 2, nf);
+  }
+  L.n_seg = L.n_trav = L.n_nf = 0;
+}
+
+template <bool ADJ>
+__device__ __forceinline__ void path_loop(const PathArgs& P, int* stk, float* lds_grad) {
+  const int stride = blockDim.x;
+  const int lane = (int)__lane_id();
+  // wave-uniform job cursor: a wave takes items from one job until that
+  // job's queue is dry and none of its lanes still runs a path of it.
+  // Items come in chunks of P.chunk per atomic (a wave-local range).
+  int cur = 0;
+  bool gdry = false;
+  uint32_t lbase = 0, lend = 0;
+  Lane L;
+  L.phase = PH_PARK;  // waiting for a work item
+  L.band = 0;
+  L.pi = L.pj = L.blk = 0;
+  L.acc = 0.0f;
+  L.acc_band = 0;
+  L.n_seg = L.n_trav = L.n_nf = 0;
+  L.pass = 0;
+  L.Lw = L.Eb = L.sp_dr = L.stot_dsig = L.hg_sc = 0.0f;
+  L.fdirect = 0.0f;
+  bool has_item = false;
+  // The job table is written only by the host: read it through the constant
+  // address space so every field access is a (cached) scalar load.
+  typedef __attribute__((address_space(4))) const Job ConstJob;
+  const ConstJob* jobs = (const ConstJob*)P.jobs;
+  while (true) {
+    cur = __builtin_amdgcn_readfirstlane(cur);
+    const Job& J = *(const Job*)(jobs + cur);
+    const Ctx<ADJ> c{J, P.adj, lds_grad};
+    if (L.phase == PH_NEW) {
+      L.band += 1;
+      L.pass = 0;
+      if (L.band >= J.s.B) {  // item finished: count it, stamp the job's completion
+        if (has_item) {
+          has_item = false;
+          uint32_t d = atomicAdd(P.done_items + cur, 1u) + 1u;
+          if (d == J.n_items) P.clocks[1 + cur] = wall_clock64();
+        }
+        L.phase = PH_PARK;
+      }
+    }
+    const bool need = L.phase == PH_PARK;
+    const unsigned long long mask = __ballot(need);
+    if (mask && !(gdry && lbase >= lend)) {
+      const uint32_t n = (uint32_t)__popcll(mask);
+      const uint32_t left = lend - lbase;
+      uint32_t nb = 0, nend = 0;
+      if (left < n && !gdry) {
+        const int leader = __ffsll((long long)mask) - 1;
+        uint32_t got = 0;
+        if (lane == leader) got = atomicAdd(P.queues + cur, P.chunk);
+        nb = __builtin_amdgcn_readlane(got, leader);
+        nend = nb >= J.n_items ? nb : min(nb + P.chunk, J.n_items);
+        if (nb + P.chunk >= J.n_items) gdry = true;
+      }
+      if (need) {
+        const uint32_t r =
+            __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+        uint32_t item = 0xFFFFFFFFu;
+        if (r < left) item = lbase + r;
+        else if (nb + (r - left) < nend) item = nb + (r - left);
+        if (item != 0xFFFFFFFFu) {
+          item_to_pixel(J, item, L.pi, L.pj, L.blk);
+          L.band = 0;
+          has_item = true;
+          L.phase = PH_NEW;
+        }
+      }
+      if (left >= n) {
+        lbase += n;
+      } else {
+        lbase = min(nb + (n - left), nend);
+        lend = nend;
+      }
+    }
+    if (L.phase == PH_NEW || L.phase == PH_REPLAY) start_path<ADJ>(L, c);
+    if (__ballot(L.phase != PH_PARK) == 0ull) {
+      // every lane is parked: the job's queue is dry for this wave and none
+      // of its lanes still runs a path of it -- move the wave to the next job
+      if (!(gdry && lbase >= lend)) continue;
+      flush_counters(P, cur, L, lane);
+      if (cur + 1 >= P.n_jobs) break;
+      cur += 1;
+      gdry = false;
+      lbase = lend = 0;
+      continue;
+    }
+    if (L.phase == PH_EXT || L.phase == PH_SHD) {
+      const bool ext = L.phase == PH_EXT;
+      f3 ro = ext ? L.pos : L.pos + L.sdir * L.soff;
+      f3 rd = ext ? L.dir : L.sdir;
+      const Node4* nodes = J.s.nodes;
+      const float4* tris = J.s.tris;
+      HitD h = trace_closest(nodes, tris, ro, rd, stk, stride, P.overflow);
+      if (ext) on_extension<ADJ>(L, c, h);
+      else on_shadow<ADJ>(L, c, h);
+    }
+    if (L.phase == PH_FIN) finish_segment<ADJ>(L, c, L.fdirect);
   }
 }
 
 __global__ __launch_bounds__(256) void sr_path_kernel(PathArgs a) {
-  __shared__ int lds_stack[SR_STACK_SIZE * 256];
-  path_loop<false>(a, lds_stack + threadIdx.x, nullptr);
+  extern __shared__ int lds_dyn[];
+  if (threadIdx.x == 0) atomicMin(a.clocks, wall_clock64());
+  path_loop<false>(a, lds_dyn + threadIdx.x, nullptr);
 }
 
 __global__ __launch_bounds__(256) void sr_adjoint_kernel(PathArgs a) {
-  __shared__ int lds_stack[SR_STACK_SIZE * 256];
-  __shared__ float lds_grad[SR_MAX_PARAMS];
+  extern __shared__ int lds_dyn[];
+  float* lds_grad = reinterpret_cast<float*>(lds_dyn + SR_STACK_SIZE * blockDim.x);
   for (int i = threadIdx.x; i < a.adj.n_params; i += blockDim.x) lds_grad[i] = 0.0f;
   __syncthreads();
-  path_loop<true>(a, lds_stack + threadIdx.x, lds_grad);
+  path_loop<true>(a, lds_dyn + threadIdx.x, lds_grad);
   __syncthreads();
   for (int i = threadIdx.x; i < a.adj.n_params; i += blockDim.x) {
     float v = lds_grad[i];
@@ -743,27 +817,50 @@ __global__ __launch_bounds__(256) void sr_adjoint_kernel(PathArgs a) {
   }
 }
 
-static int grid_for(const void* fn, int* grid_out) {
+__global__ void sr_fill_u32_kernel(uint32_t* __restrict__ p, size_t n, uint32_t v) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t step = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += step) p[i] = v;
+}
+
+size_t sr_path_lds_bytes(int block, bool adjoint) {
+  return (size_t)SR_STACK_SIZE * block * sizeof(int) + (adjoint ? SR_MAX_PARAMS * sizeof(float) : 0);
+}
+
+static int grid_for(const void* fn, int block, size_t lds, int* grid_out) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return -1;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return -1;
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0) != hipSuccess) return -1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, lds) != hipSuccess) return -1;
   if (per_cu < 1) per_cu = 1;
   *grid_out = per_cu * prop.multiProcessorCount;
   return 0;
 }
 
-int sr_path_grid(int* grid_out) { return grid_for((const void*)sr_path_kernel, grid_out); }
-int sr_adjoint_grid(int* grid_out) { return grid_for((const void*)sr_adjoint_kernel, grid_out); }
+int sr_path_grid_bs(int block, int* grid_out) {
+  return grid_for((const void*)sr_path_kernel, block, sr_path_lds_bytes(block, false), grid_out);
+}
+int sr_path_grid(int* grid_out) { return sr_path_grid_bs(256, grid_out); }
+int sr_adjoint_grid(int* grid_out) {
+  return grid_for((const void*)sr_adjoint_kernel, 256, sr_path_lds_bytes(256, true), grid_out);
+}
 
-int sr_launch_path(const PathArgs& a, int grid, hipStream_t st) {
-  hipLaunchKernelGGL(sr_path_kernel, dim3(grid), dim3(256), 0, st, a);
+int sr_launch_path(const PathArgs& a, int grid, int block, hipStream_t st) {
+  hipLaunchKernelGGL(sr_path_kernel, dim3(grid), dim3(block), sr_path_lds_bytes(block, false), st, a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int sr_launch_adjoint(const PathArgs& a, int grid, hipStream_t st) {
-  hipLaunchKernelGGL(sr_adjoint_kernel, dim3(grid), dim3(256), 0, st, a);
+int sr_launch_adjoint(const PathArgs& a, int grid, int block, hipStream_t st) {
+  hipLaunchKernelGGL(sr_adjoint_kernel, dim3(grid), dim3(block), sr_path_lds_bytes(block, true), st, a);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int sr_launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t st) {
+  if (n == 0) return 0;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(sr_fill_u32_kernel, dim3((unsigned)blocks), dim3(256), 0, st, p, n, v);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

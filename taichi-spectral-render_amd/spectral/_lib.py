@@ -13,7 +13,8 @@ from pathlib import Path
 
 import torch  # noqa: F401  (maps the HIP runtime first; see module docstring)
 
-LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libspectral_render.so"
+LIB_PATH = Path(os.environ.get("SPECTRAL_RENDER_LIB",
+                               Path(__file__).resolve().parent / "_lib" / "libspectral_render.so"))
 
 SR_DEVICE_PTRS = 1
 ERRORS = {-1: "EINVAL", -2: "ESTATE", -3: "EHIP", -4: "ENOMEM", -5: "EOVERFLOW"}
@@ -39,13 +40,21 @@ SIGNATURES = {
                    ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
     "sr_render_adjoint": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int,
                            ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                           ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
+                           ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
     "sr_resolve": ([_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                     ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
     "sr_stats": ([_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                   ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
     "sr_last_kernel_ms": ([_vp, _c_float_p], ctypes.c_int),
     "sr_sync": ([_vp], ctypes.c_int),
+    "sr_batch_create": ([ctypes.POINTER(_vp)], ctypes.c_int),
+    "sr_batch_destroy": ([_vp], ctypes.c_int),
+    "sr_batch_submit": ([_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int,
+                         ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
+    "sr_batch_clear": ([_vp], ctypes.c_int),
+    "sr_batch_config": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "sr_batch_launch": ([_vp, ctypes.c_int], ctypes.c_int),
+    "sr_batch_wait": ([_vp, _c_float_p, _vp, _vp, _vp], ctypes.c_int),
 }
 
 _LIB = None
@@ -66,7 +75,9 @@ def load():
         raise RuntimeError(f"HIP library not built: {LIB_PATH} is missing (run __graft_entry__.build())")
     lib = ctypes.CDLL(str(LIB_PATH), mode=os.RTLD_NOW | os.RTLD_GLOBAL)
     for name, (args, res) in SIGNATURES.items():
-        fn = getattr(lib, name)
+        fn = getattr(lib, name, None)
+        if fn is None:  # an older build: the call will fail with AttributeError
+            continue
         fn.argtypes = args
         fn.restype = res
     _LIB = lib

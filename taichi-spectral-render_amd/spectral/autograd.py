@@ -109,16 +109,19 @@ class DifferentiableScene:
         self.r.render(n_blocks=self.spp, max_depth=self.depth, seed=self.seed, stream=stream)
         return self.r.accum
 
-    def adjoint(self, d_accum, stream=0):
+    def adjoint(self, d_accum, stream=0, flags=0):
         """Gradient of sum(d_accum * render()) with respect to the parameters."""
         d = d_accum.contiguous().to(torch.float32)
+        want = (self.r.width, self.r.height, self.B)
+        if tuple(d.shape) != want or d.device.type != "cuda":
+            raise ValueError(f"d_accum must be a cuda tensor of shape {want}, got {tuple(d.shape)} on {d.device}")
         grad = torch.zeros((self.n_params,), dtype=torch.float32, device=d.device)
         sp = self.spec
         self.r._bind_stream()
         _lib.call("sr_render_adjoint", self.r._h, self.r.width, self.r.height, ctypes.c_uint64(self.seed),
                   ctypes.c_uint32(stream), 0, self.spp, self.depth, _lib.ptr(d), _lib.ptr(self.alb_off),
                   sp.rough_mat, self.offsets.get("roughness", -1), sp.g_mat, self.offsets.get("g", -1),
-                  sp.sigma_mat, self.offsets.get("sigma_a", -1), self.n_params, _lib.ptr(grad))
+                  sp.sigma_mat, self.offsets.get("sigma_a", -1), self.n_params, int(flags), _lib.ptr(grad))
         return grad
 
 

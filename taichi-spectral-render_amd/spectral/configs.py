@@ -131,11 +131,14 @@ _ICO_F = [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], 
           [8, 6, 7], [9, 8, 1]]
 
 
-def icosphere(center, radius, material_id, subdivisions=2, displace=None):
-    """create_sphere, main.py:195-237: f32 icosahedron, midpoint subdivision
-    with a shared-edge midpoint cache, vertex normals = unit positions.
-    ``displace`` (optional) maps unit vertex directions [V,3] to radial
-    scale factors (used by the batch scenes)."""
+_UNIT_ICO = {}
+
+
+def _unit_icosphere(subdivisions):
+    """Unit-sphere vertices and faces of create_sphere's subdivision
+    (main.py:209-233), memoised per level."""
+    if subdivisions in _UNIT_ICO:
+        return _UNIT_ICO[subdivisions]
     verts = [np.array(v, np.float32) for v in _ICO_V]
     verts = [v / np.float32(np.linalg.norm(v)) for v in verts]
     faces = [list(f) for f in _ICO_F]
@@ -157,13 +160,23 @@ def icosphere(center, radius, material_id, subdivisions=2, displace=None):
             new_faces.append([v3, mid[m31], mid[m23]])
             new_faces.append([mid[m12], mid[m23], mid[m31]])
         faces = new_faces
-    V = np.stack(verts).astype(np.float32)
+    out = (np.stack(verts).astype(np.float32), np.asarray(faces, np.int64))
+    _UNIT_ICO[subdivisions] = out
+    return out
+
+
+def icosphere(center, radius, material_id, subdivisions=2, displace=None):
+    """create_sphere, main.py:195-237: f32 icosahedron, midpoint subdivision
+    with a shared-edge midpoint cache, vertex normals = unit positions.
+    ``displace`` (optional) maps unit vertex directions [V,3] to radial
+    scale factors (used by the batch scenes)."""
+    V, F = _unit_icosphere(subdivisions)
     normals = V / np.linalg.norm(V, axis=1, keepdims=True)
     scale = np.float32(radius)
     if displace is not None:
         scale = (np.float32(radius) * np.asarray(displace(V), np.float32))[:, None]
     final = V * scale + np.asarray(center, np.float32)
-    out = {"vertices": final, "faces": np.asarray(faces, np.int64), "material_id": material_id}
+    out = {"vertices": final, "faces": F.copy(), "material_id": material_id}
     if displace is None:
         out["vertex_normals"] = normals.astype(np.float32)
     return out

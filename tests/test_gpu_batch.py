@@ -1,0 +1,71 @@
+"""Batched multi-scene launch (sr_batch_*): one persistent kernel over
+several scenes must reproduce each scene's own sr_render result (same RNG
+keys per path), with and without hipGraph capture/replay."""
+import numpy as np
+import pytest
+
+from spectral import configs
+from spectral.scene import flatten_scene
+
+pytestmark = pytest.mark.gpu
+
+
+def _scenes():
+    c0 = configs.box_icosphere(width=32, height=32, spp=4)
+    c1 = configs.heightfield_1m(width=24, height=16, spp=3, nx=64, nz=32)
+    c2 = configs.volume_fluor(width=16, height=16, spp=2)
+    out = []
+    for c in (c0, c1, c2):
+        out.append((c, flatten_scene(c.scene, c.bands, c.wl_min, c.wl_max)))
+    return out
+
+
+def _renderers(gpu):
+    from spectral.render import Renderer
+
+    return [(c, Renderer(f, c.bands, c.wl_min, c.wl_max, width=c.width, height=c.height, device=gpu))
+            for c, f in _scenes()]
+
+
+def test_batch_matches_sequential(gpu):
+    from spectral.batch import Batch
+
+    seq = _renderers(gpu)
+    ref, segs = [], []
+    for c, r in seq:
+        ref.append(r.render(n_blocks=c.spp, max_depth=c.depth, seed=c.seed).cpu().numpy())
+        segs.append(r.stats()["segments"])
+    bat = _renderers(gpu)
+    b = Batch()
+    for c, r in bat:
+        b.submit(r, c.spp, c.depth, c.seed)
+    for graph in (False, True, True):
+        for _, r in bat:
+            r.clear_pixel_space()
+        out = b.run(graph=graph)
+        for k, (c, r) in enumerate(bat):
+            got = r.accum.cpu().numpy()
+            assert np.allclose(got, ref[k], rtol=1e-5, atol=1e-6), (graph, c.name)
+        assert list(out["segments"]) == segs
+        lat = out["latency_ms"]
+        assert np.all(lat > 0) and lat.max() <= out["total_ms"] * 1.05 + 0.5
+
+
+def test_batch_config_and_tuner(gpu):
+    from spectral.batch import Batch, tune
+
+    bat = _renderers(gpu)
+    spec = [(r, 1, c.depth, c.seed) for c, r in bat]
+    best, timings = tune(spec, blocks=(128, 256), grid_pcts=(100,), chunks=(64, 512), reps=1)
+    assert best in timings and len(timings) == 4
+    b = Batch()
+    with pytest.raises(Exception):
+        b.set_config(96, 100, 64)
+    b.set_config(*best)
+    for c, r in bat:
+        r.clear_pixel_space()
+        b.submit(r, 1, c.depth, c.seed)
+    with pytest.raises(Exception):
+        b.submit(bat[0][1], 1, 4, 0)  # a handle may appear once per batch
+    out = b.run()
+    assert all(s > 0 for s in out["segments"])

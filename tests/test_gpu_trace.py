@@ -56,3 +56,19 @@ def test_device_pointer_path(box):
     t0, p0, _ = ora.trace(o, d)
     assert np.array_equal(p.cpu().numpy(), p0)
     assert np.array_equal(t.cpu().numpy(), t0)
+
+
+def test_single_triangle_leaf_is_hit(gpu):
+    # One face: its leaf is (first = 0, count = 1), the encoding that once
+    # collided with the empty-slot sentinel.
+    from spectral.render import Renderer
+
+    B = 4
+    scene = {"materials": [configs._mat(B, n=1.0, tv=1), configs._mat(B, n=1.4, ss=1.0)],
+             "meshes": [{"vertices": [[-1, -1, 0], [1, -1, 0], [0, 1, 0]], "faces": [[0, 1, 2]], "material_id": 1}],
+             "camera": {"origin": [0, 0, 2], "lookat": [0, 0, 0], "up": [0, 1, 0]}}
+    flat = flatten_scene(scene, B, 400.0, 700.0)
+    r = Renderer(flat, B, 400.0, 700.0, device=gpu)
+    t, p, _ = r.trace(np.array([[0, 0, 2], [5, 5, 2]], np.float32), np.array([[0, 0, -1], [0, 0, -1]], np.float32))
+    assert list(p) == [0, -1]
+    assert t[0] == np.float32(2.0)
