@@ -34,6 +34,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <vector>
@@ -97,20 +99,44 @@ static inline void philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in
   out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
-// One path's uniform stream: call n of the path returns word (n & 3) of
-// Philox(ctr = {c0, c1, c2, n >> 2}, key = seed).  Stands in for ti.random().
+// Uniforms standing in for ti.random().  A path draws them in fixed blocks:
+// block (segment k, slot s) is Philox(ctr = {c0, c1, c2, k << 3 | s},
+// key = seed), four words mapped to [0, 1) with 24 bits each, and each
+// sampling step of a segment owns one slot (the step's draws are words
+// 0, 1, ... of its block in call order).  The reference draws one stream per
+// thread (main.py:9, ti.random); any assignment of independent uniforms to
+// the draws gives the same estimator, and this one lets the GPU kernel form
+// a step's block once for all lanes that reach the step.
+// Without a slot the generator is a plain stream: call n returns word n & 3
+// of block n >> 2.
+enum { SLOT_EXT = 0, SLOT_NEE = 1, SLOT_SCAT = 2, SLOT_OPQ = 3, SLOT_AUX = 4 };
 struct Rng {
   uint32_t key[2];
   uint32_t c0, c1, c2;
-  uint32_t n = 0;
+  uint32_t n = 0;    // plain stream: words drawn; slotted: words drawn from the block
+  uint32_t seg = 0;  // path segment of the slotted blocks
+  bool slotted = false;
   uint32_t buf[4];
+  // Position the draws at word `first` of block (seg, sl).
+  void slot(int sl, int first = 0) {
+    uint32_t ctr[4] = {c0, c1, c2, (seg << 3) | (uint32_t)sl};
+    philox4x32_10(ctr, key, buf);
+    slotted = true;
+    n = (uint32_t)first;
+  }
   float next() {
-    if ((n & 3u) == 0u) {
-      uint32_t ctr[4] = {c0, c1, c2, n >> 2};
-      philox4x32_10(ctr, key, buf);
+    uint32_t x;
+    if (slotted) {
+      if (n >= 4u) { std::fprintf(stderr, "oracle: more than 4 draws in one slot\n"); std::abort(); }
+      x = buf[n++];
+    } else {
+      if ((n & 3u) == 0u) {
+        uint32_t ctr[4] = {c0, c1, c2, n >> 2};
+        philox4x32_10(ctr, key, buf);
+      }
+      x = buf[n & 3u];
+      ++n;
     }
-    uint32_t x = buf[n & 3u];
-    ++n;
     return (float)(x >> 8) * (1.0f / 16777216.0f);
   }
 };
@@ -583,6 +609,7 @@ static SurfaceEvent scatter_surface(const Scene& s, const Ray& ray, const Volume
   V3 V = -ray.direction;
   V3 N = ev.shading_normal;
   float ni = vm.n, ki = vm.k, nt = sm.n, kt = sm.k;
+  rng.slot(SLOT_SCAT);
   float x0 = rng.next();
   float x1 = rng.next();
   V3 H = importance_sample_ggx(x0, x1, N, sm.r);
@@ -611,6 +638,7 @@ static SurfaceEvent scatter_surface(const Scene& s, const Ray& ray, const Volume
     float dist_to_event = -std::log(-rng.next() + 1.0f + 1e-7f) / (sm.ss + absorb + 1e-6f);
     if (!sm.tv) {
       res.did_reflect = 1;
+      rng.slot(SLOT_OPQ);
       V3 S = cosine_weighted_hemisphere_direction(N, rng);
       float NdotL = tmax(1e-4f, dot(N, S));
       float NdotV = tmax(1e-4f, dot(N, V));
@@ -700,6 +728,7 @@ static VolumeEvent scatter_volume(const Scene& s, const Ray& ray, Rng& rng, Coun
   res.direction = ray.direction;
   res.penetrated_tri_idx = ray.penetrated_tri_idx;
   float sa = (4.0f * PI_F * vm.k) / lam_m(s, a);
+  rng.slot(SLOT_EXT);
   float d_vol = -std::log(-rng.next() + 1.0f - 1e-7f) / (vm.ss + sa + 1e-6f);
   Hit hit = trace(s, ray.position, ray.direction);
   cnt.traversals++;
@@ -746,6 +775,7 @@ static VolumeEvent scatter_volume(const Scene& s, const Ray& ray, Rng& rng, Coun
             if (r2 < cum) { nw = j; break; }
           }
           res.new_wavelength_idx = nw;
+          rng.slot(SLOT_AUX, 2);
           res.direction = random_in_unit_sphere(rng);
           float em2 = ((-(1.0f - T) + 1.0f) * get_mat(s, ray.vol_mat_id, nw).em) / (sa + 1e-6f);
           res.estimator = (1.0f - T) + em2;
@@ -840,6 +870,7 @@ static NEE sample_direct_light(const Scene& s, const Ray& ray, V3 normVec, int s
   const int wl = ray.active;
   int nl = s.nlights[wl];
   if (nl > 0) {
+    rng.slot(SLOT_NEE);
     float r = rng.next();
     int lt_idx = 0;
     for (int i = 0; i < nl; ++i) {
@@ -879,6 +910,8 @@ static NEE sample_direct_light(const Scene& s, const Ray& ray, V3 normVec, int s
 // Contributions are added to acc[band] (the pixel's B-vector).
 static void cast_ray_path(const Scene& s, int W, int H, int i, int j, int band, Rng& rng, int depth, float* acc,
                           Counters& cnt) {
+  rng.seg = 0;
+  rng.slot(SLOT_AUX);
   float u = ((float)i + rng.next()) / (float)W;
   float v = ((float)j + rng.next()) / (float)H;
   const float aspect = (float)W / (float)H;
@@ -896,6 +929,7 @@ static void cast_ray_path(const Scene& s, int W, int H, int i, int j, int band, 
   int active_field = band;  // active_wavelength_idx[i, j]
   for (int k = 0; k < depth; ++k) {
     if (ray.has_terminated) break;
+    rng.seg = (uint32_t)k;
     // A non-finite throughput (e.g. the 0/0 rough BTDF estimator when a path
     // leaves a rough dielectric, main.py:1176-1185 with n_i == n_t) would
     // make every later contribution NaN; the path ends here instead.

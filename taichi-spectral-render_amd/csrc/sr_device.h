@@ -47,9 +47,12 @@ __device__ __forceinline__ float clampf(float x, float lo, float hi) { return tm
 __device__ __forceinline__ f3 xyz(float4 v) { return mk(v.x, v.y, v.z); }
 
 // ------------------------------------------------------------------ RNG
-// Philox-4x32-10 (Salmon et al., SC'11).  Path stream: uniform n is word
-// (n & 3) of Philox(ctr = {c0, c1, c2, n >> 2}, key = seed), mapped to
-// [0, 1) with 24 bits.  Stands in for ti.random().
+// Philox-4x32-10 (Salmon et al., SC'11), standing in for ti.random().  A
+// path's uniforms come in fixed blocks: block (segment k, slot s) is
+// Philox(ctr = {pixel, block, band | stream << 8, k << 3 | s}, key = seed),
+// its four words mapped to [0, 1) with 24 bits each.  Each sampling step of
+// a segment owns one slot (SR_SLOT_*), so a handler generates its block once,
+// at one place in the code, for every lane that reaches it.
 __device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
                                        uint32_t out[4]) {
 #pragma unroll
@@ -64,24 +67,23 @@ __device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, ui
   out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
-struct Rng {
-  uint32_t k0, k1, c0, c1, c2, n;
-  uint32_t b0, b1, b2, b3;
-  __device__ __forceinline__ void init(uint32_t key0, uint32_t key1, uint32_t a, uint32_t b, uint32_t c) {
-    k0 = key0; k1 = key1; c0 = a; c1 = b; c2 = c; n = 0;
-  }
-  __device__ __forceinline__ float next() {
-    if ((n & 3u) == 0u) {
-      uint32_t o[4];
-      philox(c0, c1, c2, n >> 2, k0, k1, o);
-      b0 = o[0]; b1 = o[1]; b2 = o[2]; b3 = o[3];
-    }
-    uint32_t w = n & 3u;
-    uint32_t x = w == 0 ? b0 : (w == 1 ? b1 : (w == 2 ? b2 : b3));
-    ++n;
-    return (float)(x >> 8) * (1.0f / 16777216.0f);
-  }
+enum {
+  SR_SLOT_EXT = 0,   // scatter_volume: free flight, event choice, 2 for HG or EEM
+  SR_SLOT_NEE = 1,   // sample_direct_light: light pick, point on the light (2)
+  SR_SLOT_SCAT = 2,  // scatter_surface: GGX half vector (2), Fresnel choice, subsurface distance
+  SR_SLOT_OPQ = 3,   // scatter_surface, opaque: cosine direction (2), surface EEM (2)
+  SR_SLOT_AUX = 4    // camera jitter (segment 0, words 0-1); fluorescence direction (words 2-3)
 };
+struct U4 {
+  float u[4];
+};
+__device__ __forceinline__ float u24(uint32_t w) { return (float)(w >> 8) * (1.0f / 16777216.0f); }
+__device__ __forceinline__ U4 draw4(uint32_t pixel, uint32_t block, uint32_t band_stream, uint32_t seg, uint32_t slot,
+                                    uint32_t k0, uint32_t k1) {
+  uint32_t o[4];
+  philox(pixel, block, band_stream, (seg << 3) | slot, k0, k1, o);
+  return U4{{u24(o[0]), u24(o[1]), u24(o[2]), u24(o[3])}};
+}
 
 // ------------------------------------------------------------------ scene view
 struct SceneDev {
@@ -266,45 +268,67 @@ __device__ __forceinline__ SurfHit surface_of(const SceneDev& s, const HitD& h, 
 }
 
 // ------------------------------------------------------------------ sampling (main.py:682-792)
+// sin(2 pi u) and cos(2 pi u).  The turn fraction is reduced exactly:
+// r = u - rint(u), quadrant q = rint(4 r), f = r - q/4 in [-1/8, 1/8]; both
+// functions are then the Taylor series of sin(2 pi f) (to f^9) and
+// cos(2 pi f) (to f^10) -- truncation below 2e-9 on that interval, about
+// 2 ulp after rounding -- rotated by q quarter turns.  The sampling angles
+// of the reference are all 2 pi * uniform, so this replaces sinf/cosf of
+// the rounded product with the same functions of the exact turn count.
+__device__ __forceinline__ void sincos_turns(float u, float& s, float& c) {
+  const float r = u - rintf(u);
+  const float q = rintf(4.0f * r);
+  const float f = fmaf(-0.25f, q, r);
+  const float f2 = f * f;
+  const float ps = fmaf(fmaf(fmaf(fmaf(42.058693f, f2, -76.70586f), f2, 81.60525f), f2, -41.3417f), f2, 6.2831855f) * f;
+  const float pc =
+      fmaf(fmaf(fmaf(fmaf(fmaf(-26.426256f, f2, 60.24464f), f2, -85.45682f), f2, 64.93939f), f2, -19.739208f), f2, 1.0f);
+  const int qi = ((int)q) & 3;
+  const float sa = (qi & 1) ? pc : ps;
+  const float ca = (qi & 1) ? ps : pc;
+  s = (qi & 2) ? -sa : sa;
+  c = ((qi + 1) & 2) ? -ca : ca;
+}
 __device__ __forceinline__ void onb(f3 n, f3& t, f3& b) {
   f3 up = fabsf(n.y) < 0.999f ? mk(0, 1, 0) : mk(1, 0, 0);
   t = normalize(cross(up, n));
   b = cross(n, t);
 }
-__device__ __forceinline__ f3 random_in_unit_sphere(Rng& rng) {
-  float z = rng.next() * 2.0f - 1.0f;
+__device__ __forceinline__ f3 random_in_unit_sphere(float u0, float u1) {
+  float z = u0 * 2.0f - 1.0f;
   float rxy = sqrtf(tmax(0.0f, 1.0f - z * z));
-  float phi = rng.next() * 2.0f * SR_PI;
-  return mk(rxy * cosf(phi), rxy * sinf(phi), z);
+  float sp, cp;
+  sincos_turns(u1, sp, cp);  // phi = 2 pi u1
+  return mk(rxy * cp, rxy * sp, z);
 }
-__device__ __forceinline__ f3 cosine_hemisphere(f3 n, Rng& rng) {
-  float r1 = rng.next();
-  float r2 = rng.next();
-  float phi = 2.0f * SR_PI * r1;
+__device__ __forceinline__ f3 cosine_hemisphere(f3 n, float r1, float r2) {
+  float sp, cp;
+  sincos_turns(r1, sp, cp);  // phi = 2 pi r1
   float ct = sqrtf(1.0f - r2);
   float st = sqrtf(r2);
   f3 t, b;
   onb(n, t, b);
-  return t * cosf(phi) * st + b * sinf(phi) * st + n * ct;
+  return t * cp * st + b * sp * st + n * ct;
 }
 __device__ __forceinline__ float hg_eval(float c, float g) {
   float g2 = g * g;
   float den = 1.0f + g2 - 2.0f * g * c;
   return (1.0f - g2) / (4.0f * SR_PI * den * sqrtf(den) + 1e-6f);
 }
-__device__ __forceinline__ f3 hg_sample(f3 w, float g, Rng& rng) {
+__device__ __forceinline__ f3 hg_sample(f3 w, float g, float u0, float u1) {
   float ct;
   if (fabsf(g) < 1e-4f) {
-    ct = 2.0f * rng.next() - 1.0f;
+    ct = 2.0f * u0 - 1.0f;
   } else {
-    float rs = (1.0f - g * g) / (1.0f - g + 2.0f * g * rng.next());
+    float rs = (1.0f - g * g) / (1.0f - g + 2.0f * g * u0);
     ct = (1.0f + g * g - rs * rs) / (2.0f * g);
   }
   float st = sqrtf(tmax(0.0f, 1.0f - ct * ct));
-  float phi = 2.0f * SR_PI * rng.next();
+  float sp, cp;
+  sincos_turns(u1, sp, cp);  // phi = 2 pi u1
   f3 u, v;
   onb(w, u, v);
-  return u * cosf(phi) * st + v * sinf(phi) * st + w * ct;
+  return u * cp * st + v * sp * st + w * ct;
 }
 
 // ------------------------------------------------------------------ microfacet / Fresnel (main.py:866-978)
@@ -317,11 +341,14 @@ __device__ __forceinline__ cplx cdiv(cplx a, cplx b) {
   cplx p = cmul(a, cplx{b.re, -b.im});
   return cplx{p.re / m, p.im / m};
 }
+// Principal square root as complex_sqrt (main.py:884-896).  With a
+// correctly rounded sqrt mod >= |re|, so both radicands are >= 0; the clamps
+// only guard against a less exact sqrt and change no correctly rounded result.
 __device__ __forceinline__ cplx csqrt_ref(cplx z) {
   float mod = sqrtf(z.re * z.re + z.im * z.im);
-  float re = sqrtf((mod + z.re) / 2.0f);
+  float re = sqrtf(tmax(0.0f, (mod + z.re) / 2.0f));
   float sg = z.im > 0.f ? 1.f : (z.im < 0.f ? -1.f : 0.f);
-  float im = sg * sqrtf((mod - z.re) / 2.0f);
+  float im = sg * sqrtf(tmax(0.0f, (mod - z.re) / 2.0f));
   if (z.im == 0.0f) im = 0.0f;
   return cplx{re, im};
 }
@@ -343,12 +370,11 @@ __device__ __forceinline__ float ggx_G(f3 n, f3 v, f3 l, float r) {
 }
 __device__ __forceinline__ float fresnel_spectral(float ci, float ni, float ki, float nt, float kt) {
   float c = clampf(ci, -1.0f, 1.0f);
-  float ang = acosf(c);
   cplx ein{ni * ni - ki * ki, 2 * ni * ki};
   cplx etr{nt * nt - kt * kt, 2 * nt * kt};
-  float ct = cosf(ang);
-  float sn = sinf(ang);
-  float s2 = sn * sn;
+  // the reference forms cos and sin^2 of acos(c) (main.py:929-943): c, 1 - c^2
+  float ct = c;
+  float s2 = 1.0f - c * c;
   cplx left = cmul(csqrt_ref(ein), cplx{ct, 0.0f});
   cplx em = cmul(ein, cplx{s2, 0.0f});
   cplx right = csqrt_ref(cplx{etr.re - em.re, etr.im - em.im});
@@ -360,25 +386,33 @@ __device__ __forceinline__ float fresnel_spectral(float ci, float ni, float ki, 
 }
 __device__ __forceinline__ f3 ggx_sample_half(float x0, float x1, f3 n, float alpha) {
   float a2 = alpha * alpha;
-  float phi = 2.0f * SR_PI * x0;
+  float sp, cp;
+  sincos_turns(x0, sp, cp);  // phi = 2 pi x0
   float ct = sqrtf((1.0f - x1) / (1.0f + (a2 - 1.0f) * x1));
-  float st = sqrtf(1.0f - ct * ct);
+  float st = sqrtf(tmax(0.0f, 1.0f - ct * ct));  // ct may round one ulp above 1
   f3 t, b;
   onb(n, t, b);
-  return normalize(t * (cosf(phi) * st) + b * (sinf(phi) * st) + n * ct);
+  return normalize(t * (cp * st) + b * (sp * st) + n * ct);
+}
+// cos(phi_i - phi_o) * sin(alpha) * tan(beta) of oren_nayar (main.py:1071-1088),
+// alpha = max(acos nl, acos nv), beta = min(...).  The larger angle has the
+// smaller cosine, so sin(alpha) = sqrt(1 - cmin^2), tan(beta) =
+// sqrt(1 - cmax^2) / cmax.  A cosine above 1 (acos -> NaN) drops out of the
+// max/min as NaN operands do there; both above 1 gives NaN.  The term is
+// only needed when both cosines exceed 1e-4 (otherwise cos(phi) = 0).
+__device__ __forceinline__ float oren_nayar_angular(float nl, float nv, f3 l, f3 v, f3 n) {
+  if (!(nl > 1e-4f && nv > 1e-4f)) return 0.0f;
+  f3 pi = normalize(l - n * nl);
+  f3 po = normalize(v - n * nv);
+  const float cpd = tmax(0.0f, dot(pi, po));
+  const float ca = nl > 1.0f ? nv : nl, cb = nv > 1.0f ? nl : nv;
+  const float cmin = tmin(ca, cb), cmax = tmax(ca, cb);
+  return cpd * (sqrtf(1.0f - cmin * cmin) * sqrtf(1.0f - cmax * cmax) / cmax);
 }
 __device__ __forceinline__ float oren_nayar(float r, float nl, float nv, f3 l, f3 v, f3 n) {
   float sigma = r * SR_PI / 2;
   float s2 = sigma * sigma;
   float A = 1.0f - s2 / (2.0f * (s2 + 0.33f));
   float B = 0.45f * s2 / (s2 + 0.09f);
-  float ti = acosf(nl), to = acosf(nv);
-  float al = tmax(ti, to), be = tmin(ti, to);
-  float cpd = 0.0f;
-  if (nl > 1e-4f && nv > 1e-4f) {
-    f3 pi = normalize(l - n * nl);
-    f3 po = normalize(v - n * nv);
-    cpd = tmax(0.0f, dot(pi, po));
-  }
-  return A + B * cpd * sinf(al) * tanf(be);
+  return A + B * oren_nayar_angular(nl, nv, l, v, n);
 }

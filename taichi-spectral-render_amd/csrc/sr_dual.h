@@ -120,15 +120,7 @@ __device__ __forceinline__ Dl oren_nayar_d(Dl r, float nl, float nv, f3 l, f3 v,
   Dl s2 = sigma * sigma;
   Dl A = 1.0f - s2 / ((s2 + 0.33f) * 2.0f);
   Dl Bc = (s2 * 0.45f) / (s2 + 0.09f);
-  float ti = acosf(nl), to = acosf(nv);
-  float al = tmax(ti, to), be = tmin(ti, to);
-  float cpd = 0.0f;
-  if (nl > 1e-4f && nv > 1e-4f) {
-    f3 pi = normalize(l - n * nl);
-    f3 po = normalize(v - n * nv);
-    cpd = tmax(0.0f, dot(pi, po));
-  }
-  return A + Bc * (cpd * sinf(al) * tanf(be));
+  return A + Bc * oren_nayar_angular(nl, nv, l, v, n);
 }
 __device__ __forceinline__ Dl hg_eval_d(float c, Dl g) {  // main.py:758-766
   Dl g2 = g * g;

@@ -29,7 +29,6 @@ enum { PH_NEW = 0, PH_EXT = 1, PH_SHD = 2, PH_REPLAY = 3, PH_PARK = 4, PH_FIN = 
 
 struct Lane {
   int phase, band, pi, pj, blk;
-  Rng rng;
   f3 pos, dir;
   float tp, sp;
   int active, pl, vol, seg;
@@ -56,6 +55,13 @@ struct Ctx {
   const AdjMap& adj;
   float* lds_grad;
 };
+
+// The lane's uniform block (segment `seg`, slot `slot`); the key is the job's
+// seed (wave-uniform, so the key schedule runs on the scalar unit).
+__device__ __forceinline__ U4 draws(const Job& a, const Lane& L, int seg, int slot) {
+  return draw4((uint32_t)(L.pi * a.H + L.pj), (uint32_t)L.blk, (uint32_t)L.band | (a.stream << 8), (uint32_t)seg,
+               (uint32_t)slot, a.seed_lo, a.seed_hi);
+}
 
 template <bool ADJ>
 __device__ __forceinline__ float dacc(const Ctx<ADJ>& c, const Lane& L, int band) {
@@ -114,10 +120,9 @@ template <bool ADJ>
 __device__ __forceinline__ void start_path(Lane& L, const Ctx<ADJ>& c) {
   const Job& a = c.a;
   const SceneDev& s = a.s;
-  L.rng.init(a.seed_lo, a.seed_hi, (uint32_t)(L.pi * a.H + L.pj), (uint32_t)L.blk,
-             (uint32_t)L.band | (a.stream << 8));
-  float u = ((float)L.pi + L.rng.next()) / (float)a.W;
-  float v = ((float)L.pj + L.rng.next()) / (float)a.H;
+  const U4 r = draws(a, L, 0, SR_SLOT_AUX);
+  float u = ((float)L.pi + r.u[0]) / (float)a.W;
+  float v = ((float)L.pj + r.u[1]) / (float)a.H;
   const float aspect = (float)a.W / (float)a.H;
   f3 cu = mk(s.cam_u[0], s.cam_u[1], s.cam_u[2]);
   f3 cv = mk(s.cam_v[0], s.cam_v[1], s.cam_v[2]);
@@ -219,12 +224,13 @@ __device__ __forceinline__ void scatter_surface(Lane& L, const Ctx<ADJ>& c, floa
   const f3 V = -L.dir;
   const f3 N = L.esn;
   const float ni = vm.n, ki = vm.k, nt = sm.n, kt = sm.k;
-  float x0 = L.rng.next();
-  float x1 = L.rng.next();
+  const U4 rs = draws(a, L, L.seg, SR_SLOT_SCAT);
+  float x0 = rs.u[0];
+  float x1 = rs.u[1];
   f3 H = ggx_sample_half(x0, x1, N, sm.r);
   float ia = tmax(1e-4f, dot(V, H));
   float F = fresnel_spectral(ia, ni, ki, nt, kt);
-  float rnd = clampf(L.rng.next(), 0.0f, 0.99f);
+  float rnd = clampf(rs.u[2], 0.0f, 0.99f);
   int dr = 0;
   f3 ndir;
   float est, pdf;
@@ -241,11 +247,12 @@ __device__ __forceinline__ void scatter_surface(Lane& L, const Ctx<ADJ>& c, floa
     else branch = 1;
   } else {
     float absorb = (4.0f * SR_PI * kt) / s.lam_m[ab];
-    float dte = -logf(-L.rng.next() + 1.0f + 1e-7f) / (sm.ss + absorb + 1e-6f);
+    float dte = -logf(-rs.u[3] + 1.0f + 1e-7f) / (sm.ss + absorb + 1e-6f);
     if (!sm.tv) {
       dr = 1;
       branch = 3;
-      f3 S = cosine_hemisphere(N, L.rng);
+      const U4 ro = draws(a, L, L.seg, SR_SLOT_OPQ);
+      f3 S = cosine_hemisphere(N, ro.u[0], ro.u[1]);
       float NdotL = tmax(1e-4f, dot(N, S));
       float NdotV = tmax(1e-4f, dot(N, V));
       float albedo = sm.ss / (sm.ss + absorb + 1e-6f);
@@ -257,11 +264,11 @@ __device__ __forceinline__ void scatter_surface(Lane& L, const Ctx<ADJ>& c, floa
         const float* row = s.eem + ((size_t)eid * s.B + ab) * s.B;
         float qy = 0.0f;
         for (int j = 0; j < s.B; ++j) qy = row[j] > qy ? row[j] : qy;
-        float r3 = L.rng.next();
+        float r3 = ro.u[2];
         if (qy > 0.0f && r3 < qy) {
           float norm = 0.0f;
           for (int j = 0; j < s.B; ++j) norm += row[j];
-          float r4 = L.rng.next();
+          float r4 = ro.u[3];
           float cum = 0.0f;
           int nw = ab;
           for (int j = 0; j < s.B; ++j) {
@@ -410,7 +417,8 @@ __device__ __forceinline__ void on_extension(Lane& L, const Ctx<ADJ>& c, const H
   const int ab = L.active;
   MatRec vm = get_mat(s, L.vol, ab);
   float sa = (4.0f * SR_PI * vm.k) / s.lam_m[ab];
-  float dvol = -logf(-L.rng.next() + 1.0f - 1e-7f) / (vm.ss + sa + 1e-6f);
+  const U4 re4 = draws(a, L, L.seg, SR_SLOT_EXT);
+  float dvol = -logf(-re4.u[0] + 1.0f - 1e-7f) / (vm.ss + sa + 1e-6f);
   float ht = h.t;
   f3 hn = mk(0.f, 0.f, 0.f), hsn = mk(0.f, 0.f, 0.f);
   int hfront = 0, hmat = 0;
@@ -444,7 +452,7 @@ __device__ __forceinline__ void on_extension(Lane& L, const Ctx<ADJ>& c, const H
     L.emat = hmat;
   } else if (travel >= 0.0f) {
     L.emat = -1;
-    float re = L.rng.next();
+    float re = re4.u[1];
     float st = sa / (vm.ss + sa + 1e-6f);
     if (re <= st) {
       absorb_branch = true;
@@ -453,12 +461,12 @@ __device__ __forceinline__ void on_extension(Lane& L, const Ctx<ADJ>& c, const H
         const float* row = s.eem + ((size_t)eid * s.B + ab) * s.B;
         float qy = 0.0f;
         for (int j = 0; j < s.B; ++j) qy = row[j] > qy ? row[j] : qy;
-        float rf = L.rng.next();
+        float rf = re4.u[2];
         if (qy > 0.0f && rf < qy) {
           L.eaction = 3;
           float norm = 0.0f;
           for (int j = 0; j < s.B; ++j) norm += row[j];
-          float r2 = L.rng.next();
+          float r2 = re4.u[3];
           float cum = 0.0f;
           int nw = ab;
           for (int j = 0; j < s.B; ++j) {
@@ -466,7 +474,8 @@ __device__ __forceinline__ void on_extension(Lane& L, const Ctx<ADJ>& c, const H
             if (r2 < cum) { nw = j; break; }
           }
           L.enew = nw;
-          L.edir = random_in_unit_sphere(L.rng);
+          const U4 rx = draws(a, L, L.seg, SR_SLOT_AUX);
+          L.edir = random_in_unit_sphere(rx.u[2], rx.u[3]);
           em_new = get_mat(s, L.vol, nw).em;
           float em2 = ((-(1.0f - T) + 1.0f) * em_new) / (sa + 1e-6f);
           est = (1.0f - T) + em2;
@@ -478,7 +487,7 @@ __device__ __forceinline__ void on_extension(Lane& L, const Ctx<ADJ>& c, const H
       }
     } else {
       L.eaction = 2;
-      L.edir = hg_sample(L.dir, vm.g, L.rng);
+      L.edir = hg_sample(L.dir, vm.g, re4.u[2], re4.u[3]);
       est = atten;
     }
   } else {
@@ -550,7 +559,8 @@ __device__ __forceinline__ void on_extension(Lane& L, const Ctx<ADJ>& c, const H
     return;
   }
   const LightRec* lr = s.lights + (size_t)ab * s.max_lights;
-  float r = L.rng.next();
+  const U4 rn = draws(a, L, L.seg, SR_SLOT_NEE);
+  float r = rn.u[0];
   int e = 0;
   for (int i = 0; i < nl; ++i) {
     if (r < lr[i].cdf) { e = i; break; }
@@ -558,7 +568,7 @@ __device__ __forceinline__ void on_extension(Lane& L, const Ctx<ADJ>& c, const H
   LightRec le = lr[e];
   const float4* lv = s.lverts + (size_t)le.slot * 3;
   f3 v0 = xyz(lv[0]), v1 = xyz(lv[1]), v2 = xyz(lv[2]);
-  float bx = L.rng.next(), by = L.rng.next();
+  float bx = rn.u[1], by = rn.u[2];
   if (bx + by > 1.0f) { bx = 1.0f - bx; by = 1.0f - by; }
   f3 p = v0 + bx * (v1 - v0) + by * (v2 - v0);
   L.sdir = normalize(p - L.pos);
@@ -705,6 +715,7 @@ __device__ __forceinline__ void path_loop(const PathArgs& P, int* stk, float* ld
   // job's queue is dry and none of its lanes still runs a path of it.
   // Items come in chunks of P.chunk per atomic (a wave-local range).
   int cur = 0;
+  int ext_streak = 0;
   bool gdry = false;
   uint32_t lbase = 0, lend = 0;
   Lane L;
@@ -784,17 +795,28 @@ __device__ __forceinline__ void path_loop(const PathArgs& P, int* stk, float* ld
       lbase = lend = 0;
       continue;
     }
-    if (L.phase == PH_EXT || L.phase == PH_SHD) {
-      const bool ext = L.phase == PH_EXT;
-      f3 ro = ext ? L.pos : L.pos + L.sdir * L.soff;
-      f3 rd = ext ? L.dir : L.sdir;
+    // One handler group per iteration, chosen for the whole wave: extension
+    // rays (trace + scatter_volume + NEE set-up) or shadow-walk steps (trace
+    // + walk step + end of segment).  Lanes of the other group wait one
+    // iteration.  Every path alternates EXT -> SHD -> FIN -> EXT, so the
+    // lanes fall into step and each handler runs once for most of the wave,
+    // instead of once per divergent subset in every iteration.  A streak
+    // limit keeps a few shadow lanes from waiting behind a wave whose
+    // extension rays keep restarting (escaping paths).
+    const int n_ext = __popcll(__ballot(L.phase == PH_EXT));
+    const int n_shd = __popcll(__ballot(L.phase == PH_SHD || L.phase == PH_FIN));
+    const bool do_ext = n_ext > n_shd && !(ext_streak >= 4 && n_shd > 0);
+    ext_streak = do_ext ? ext_streak + 1 : 0;
+    if (L.phase == (do_ext ? PH_EXT : PH_SHD)) {
+      f3 ro = do_ext ? L.pos : L.pos + L.sdir * L.soff;
+      f3 rd = do_ext ? L.dir : L.sdir;
       const Node4* nodes = J.s.nodes;
       const float4* tris = J.s.tris;
       HitD h = trace_closest(nodes, tris, ro, rd, stk, stride, P.overflow);
-      if (ext) on_extension<ADJ>(L, c, h);
+      if (do_ext) on_extension<ADJ>(L, c, h);
       else on_shadow<ADJ>(L, c, h);
     }
-    if (L.phase == PH_FIN) finish_segment<ADJ>(L, c, L.fdirect);
+    if (!do_ext && L.phase == PH_FIN) finish_segment<ADJ>(L, c, L.fdirect);
   }
 }
 
