@@ -32,7 +32,17 @@ __device__ __forceinline__ f3 cross(f3 a, f3 b) {
   return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
 __device__ __forceinline__ float length(f3 a) { return sqrtf(dot(a, a)); }
-__device__ __forceinline__ f3 normalize(f3 a) { return a / length(a); }
+// Shading arithmetic whose results feed no exact identity: the hardware
+// reciprocal, square root and reciprocal square root (about 1 ulp each;
+// rcp(0) = inf and rcp(inf) = 0, so 0/0 and x/0 keep their IEEE outcomes).
+// IEEE division and sqrt stay where the reference's exact cases live: the
+// ray/triangle test, refract (eta = 1 must return the ray), the relative
+// index n_i / n_t and the emitter estimator atten / (1 + 1e-6).
+__device__ __forceinline__ float frcp(float b) { return __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ f3 normalize(f3 a) { return a * __builtin_amdgcn_rsqf(dot(a, a)); }
+#define SR_INV_PI 0.318309886f
 __device__ __forceinline__ f3 reflect(f3 x, f3 n) { return x - 2.0f * dot(n, x) * n; }
 __device__ __forceinline__ f3 refract(f3 x, f3 n, float eta) {
   float dxn = dot(x, n);
@@ -296,7 +306,7 @@ __device__ __forceinline__ void onb(f3 n, f3& t, f3& b) {
 }
 __device__ __forceinline__ f3 random_in_unit_sphere(float u0, float u1) {
   float z = u0 * 2.0f - 1.0f;
-  float rxy = sqrtf(tmax(0.0f, 1.0f - z * z));
+  float rxy = fsqrt(tmax(0.0f, 1.0f - z * z));
   float sp, cp;
   sincos_turns(u1, sp, cp);  // phi = 2 pi u1
   return mk(rxy * cp, rxy * sp, z);
@@ -304,8 +314,8 @@ __device__ __forceinline__ f3 random_in_unit_sphere(float u0, float u1) {
 __device__ __forceinline__ f3 cosine_hemisphere(f3 n, float r1, float r2) {
   float sp, cp;
   sincos_turns(r1, sp, cp);  // phi = 2 pi r1
-  float ct = sqrtf(1.0f - r2);
-  float st = sqrtf(r2);
+  float ct = fsqrt(1.0f - r2);
+  float st = fsqrt(r2);
   f3 t, b;
   onb(n, t, b);
   return t * cp * st + b * sp * st + n * ct;
@@ -313,17 +323,17 @@ __device__ __forceinline__ f3 cosine_hemisphere(f3 n, float r1, float r2) {
 __device__ __forceinline__ float hg_eval(float c, float g) {
   float g2 = g * g;
   float den = 1.0f + g2 - 2.0f * g * c;
-  return (1.0f - g2) / (4.0f * SR_PI * den * sqrtf(den) + 1e-6f);
+  return fdiv(1.0f - g2, 4.0f * SR_PI * den * fsqrt(den) + 1e-6f);
 }
 __device__ __forceinline__ f3 hg_sample(f3 w, float g, float u0, float u1) {
   float ct;
   if (fabsf(g) < 1e-4f) {
     ct = 2.0f * u0 - 1.0f;
   } else {
-    float rs = (1.0f - g * g) / (1.0f - g + 2.0f * g * u0);
-    ct = (1.0f + g * g - rs * rs) / (2.0f * g);
+    float rs = fdiv(1.0f - g * g, 1.0f - g + 2.0f * g * u0);
+    ct = fdiv(1.0f + g * g - rs * rs, 2.0f * g);
   }
-  float st = sqrtf(tmax(0.0f, 1.0f - ct * ct));
+  float st = fsqrt(tmax(0.0f, 1.0f - ct * ct));
   float sp, cp;
   sincos_turns(u1, sp, cp);  // phi = 2 pi u1
   f3 u, v;
@@ -339,16 +349,17 @@ __device__ __forceinline__ cplx cmul(cplx a, cplx b) { return cplx{a.re * b.re -
 __device__ __forceinline__ cplx cdiv(cplx a, cplx b) {
   float m = b.re * b.re + b.im * b.im;
   cplx p = cmul(a, cplx{b.re, -b.im});
-  return cplx{p.re / m, p.im / m};
+  const float rm = frcp(m);
+  return cplx{p.re * rm, p.im * rm};
 }
 // Principal square root as complex_sqrt (main.py:884-896).  With a
 // correctly rounded sqrt mod >= |re|, so both radicands are >= 0; the clamps
 // only guard against a less exact sqrt and change no correctly rounded result.
 __device__ __forceinline__ cplx csqrt_ref(cplx z) {
-  float mod = sqrtf(z.re * z.re + z.im * z.im);
-  float re = sqrtf(tmax(0.0f, (mod + z.re) / 2.0f));
+  float mod = fsqrt(z.re * z.re + z.im * z.im);
+  float re = fsqrt(tmax(0.0f, (mod + z.re) * 0.5f));
   float sg = z.im > 0.f ? 1.f : (z.im < 0.f ? -1.f : 0.f);
-  float im = sg * sqrtf(tmax(0.0f, (mod - z.re) / 2.0f));
+  float im = sg * fsqrt(tmax(0.0f, (mod - z.re) * 0.5f));
   if (z.im == 0.0f) im = 0.0f;
   return cplx{re, im};
 }
@@ -357,15 +368,15 @@ __device__ __forceinline__ float ggx_D(float nh, float r) {
   float a2 = alpha * alpha;
   float nh2 = nh * nh;
   float den = nh2 * (a2 - 1.0f) + 1.0f;
-  return a2 / (SR_PI * den * den);
+  return fdiv(a2, SR_PI * den * den);
 }
 __device__ __forceinline__ float ggx_G(f3 n, f3 v, f3 l, float r) {
   float nv = tmax(dot(n, v), 0.0f);
   float alpha = tmax(r * r, 1e-4f);
-  float k = alpha * alpha / 2.0f;
-  float g1v = nv / (nv * (1.0f - k) + k + 1e-7f);
+  float k = alpha * alpha * 0.5f;
+  float g1v = fdiv(nv, nv * (1.0f - k) + k + 1e-7f);
   float nl = tmax(dot(n, l), 0.0f);
-  float g1l = nl / (nl * (1.0f - k) + k + 1e-7f);
+  float g1l = fdiv(nl, nl * (1.0f - k) + k + 1e-7f);
   return g1v * g1l;
 }
 __device__ __forceinline__ float fresnel_spectral(float ci, float ni, float ki, float nt, float kt) {
@@ -388,8 +399,8 @@ __device__ __forceinline__ f3 ggx_sample_half(float x0, float x1, f3 n, float al
   float a2 = alpha * alpha;
   float sp, cp;
   sincos_turns(x0, sp, cp);  // phi = 2 pi x0
-  float ct = sqrtf((1.0f - x1) / (1.0f + (a2 - 1.0f) * x1));
-  float st = sqrtf(tmax(0.0f, 1.0f - ct * ct));  // ct may round one ulp above 1
+  float ct = fsqrt(fdiv(1.0f - x1, 1.0f + (a2 - 1.0f) * x1));
+  float st = fsqrt(tmax(0.0f, 1.0f - ct * ct));  // ct may round one ulp above 1
   f3 t, b;
   onb(n, t, b);
   return normalize(t * (cp * st) + b * (sp * st) + n * ct);
@@ -407,12 +418,12 @@ __device__ __forceinline__ float oren_nayar_angular(float nl, float nv, f3 l, f3
   const float cpd = tmax(0.0f, dot(pi, po));
   const float ca = nl > 1.0f ? nv : nl, cb = nv > 1.0f ? nl : nv;
   const float cmin = tmin(ca, cb), cmax = tmax(ca, cb);
-  return cpd * (sqrtf(1.0f - cmin * cmin) * sqrtf(1.0f - cmax * cmax) / cmax);
+  return cpd * fdiv(fsqrt(1.0f - cmin * cmin) * fsqrt(1.0f - cmax * cmax), cmax);
 }
 __device__ __forceinline__ float oren_nayar(float r, float nl, float nv, f3 l, f3 v, f3 n) {
-  float sigma = r * SR_PI / 2;
+  float sigma = r * SR_PI * 0.5f;
   float s2 = sigma * sigma;
-  float A = 1.0f - s2 / (2.0f * (s2 + 0.33f));
-  float B = 0.45f * s2 / (s2 + 0.09f);
+  float A = 1.0f - fdiv(s2, 2.0f * (s2 + 0.33f));
+  float B = fdiv(0.45f * s2, s2 + 0.09f);
   return A + B * oren_nayar_angular(nl, nv, l, v, n);
 }

@@ -121,9 +121,9 @@ __device__ __forceinline__ void start_path(Lane& L, const Ctx<ADJ>& c) {
   const Job& a = c.a;
   const SceneDev& s = a.s;
   const U4 r = draws(a, L, 0, SR_SLOT_AUX);
-  float u = ((float)L.pi + r.u[0]) / (float)a.W;
-  float v = ((float)L.pj + r.u[1]) / (float)a.H;
-  const float aspect = (float)a.W / (float)a.H;
+  float u = fdiv((float)L.pi + r.u[0], (float)a.W);
+  float v = fdiv((float)L.pj + r.u[1], (float)a.H);
+  const float aspect = fdiv((float)a.W, (float)a.H);
   f3 cu = mk(s.cam_u[0], s.cam_u[1], s.cam_u[2]);
   f3 cv = mk(s.cam_v[0], s.cam_v[1], s.cam_v[2]);
   f3 cw = mk(s.cam_w[0], s.cam_w[1], s.cam_w[2]);
@@ -168,14 +168,14 @@ __device__ __forceinline__ void evaluate_bsdf(const SceneDev& s, int vol_id, int
     float NdotH = tmax(1e-4f, dot(N, H));
     float D = ggx_D(NdotH, sm.r);
     float G = ggx_G(N, V, Lv, sm.r);
-    bsdf += (D * G * F) / (4.0f * NdotL * NdotV);
-    pdf += (D * NdotH * F) / (4.0f * VdotH);
+    bsdf += fdiv(D * G * F, 4.0f * NdotL * NdotV);
+    pdf += fdiv(D * NdotH * F, 4.0f * VdotH);
   }
-  float absorb = (4.0f * SR_PI * sm.k * 1e9f) / s.lam_nm[b];
-  float albedo = sm.ss / (sm.ss + absorb + 1e-6f);
+  float absorb = fdiv(4.0f * SR_PI * sm.k * 1e9f, s.lam_nm[b]);
+  float albedo = fdiv(sm.ss, sm.ss + absorb + 1e-6f);
   float on = oren_nayar(sm.r, NdotL, NdotV, Lv, V, N);
   float opq = 1.0f - (float)sm.tv;
-  bsdf += (1.0f - F) * opq * (albedo / SR_PI) * on;
+  bsdf += (1.0f - F) * opq * (albedo * SR_INV_PI) * on;
   pdf += NdotL * 0.318309886184f * (1.0f - F) * opq;
 }
 
@@ -246,8 +246,8 @@ __device__ __forceinline__ void scatter_surface(Lane& L, const Ctx<ADJ>& c, floa
     if (sm.r < 0.008f) { est = F; pdf = 1e9f; }
     else branch = 1;
   } else {
-    float absorb = (4.0f * SR_PI * kt) / s.lam_m[ab];
-    float dte = -logf(-rs.u[3] + 1.0f + 1e-7f) / (sm.ss + absorb + 1e-6f);
+    float absorb = fdiv(4.0f * SR_PI * kt, s.lam_m[ab]);
+    float dte = fdiv(-logf(-rs.u[3] + 1.0f + 1e-7f), sm.ss + absorb + 1e-6f);
     if (!sm.tv) {
       dr = 1;
       branch = 3;
@@ -255,8 +255,8 @@ __device__ __forceinline__ void scatter_surface(Lane& L, const Ctx<ADJ>& c, floa
       f3 S = cosine_hemisphere(N, ro.u[0], ro.u[1]);
       float NdotL = tmax(1e-4f, dot(N, S));
       float NdotV = tmax(1e-4f, dot(N, V));
-      float albedo = sm.ss / (sm.ss + absorb + 1e-6f);
-      float bsdf = (1.0f - F) * (albedo / SR_PI) * oren_nayar(sm.r, NdotL, NdotV, S, V, N);
+      float albedo = fdiv(sm.ss, sm.ss + absorb + 1e-6f);
+      float bsdf = (1.0f - F) * (albedo * SR_INV_PI) * oren_nayar(sm.r, NdotL, NdotV, S, V, N);
       pdf = 0.318309886184f * NdotL;
       on_NL = NdotL; on_NV = NdotV; on_albedo = albedo; on_S = S;
       const int eid = s.mat_eem[L.emat];
@@ -271,16 +271,17 @@ __device__ __forceinline__ void scatter_surface(Lane& L, const Ctx<ADJ>& c, floa
           float r4 = ro.u[3];
           float cum = 0.0f;
           int nw = ab;
+          const float rn = frcp(norm);
           for (int j = 0; j < s.B; ++j) {
-            cum += row[j] / norm;
+            cum += row[j] * rn;
             if (r4 < cum) { nw = j; break; }
           }
           MatRec fm = get_mat(s, L.emat, nw);
-          float ab2 = (4.0f * SR_PI * fm.k) / s.lam_m[nw];
+          float ab2 = fdiv(4.0f * SR_PI * fm.k, s.lam_m[nw]);
           bsdf *= expf(-ab2 * dte);
         }
       }
-      est = (bsdf * NdotL) / pdf;
+      est = fdiv(bsdf * NdotL, pdf);
       ndir = S;
     } else {
       f3 Rf = refract(L.dir, H, ni / nt);
@@ -306,11 +307,11 @@ __device__ __forceinline__ void scatter_surface(Lane& L, const Ctx<ADJ>& c, floa
           float D = ggx_D(NdotH, sm.r);
           float den = ni * VdotH + nt * LdotH;
           float dsq = den * den;
-          float t1 = fabsf(LdotH * VdotH) / (NdotL * NdotV);
-          float t2 = (nt * nt * (1.0f - F) * D * G) / dsq;
+          float t1 = fdiv(fabsf(LdotH * VdotH), NdotL * NdotV);
+          float t2 = fdiv(nt * nt * (1.0f - F) * D * G, dsq);
           float bsdf = t1 * t2;
-          pdf = (D * NdotH * (1.0f - F) * nt * nt * fabsf(LdotH)) / dsq;
-          est = (bsdf * NdotL) / pdf;
+          pdf = fdiv(D * NdotH * (1.0f - F) * nt * nt * fabsf(LdotH), dsq);
+          est = fdiv(bsdf * NdotL, pdf);
           rf_LdotH = LdotH; rf_VdotH = VdotH; rf_NdotH = NdotH; rf_NdotL = NdotL; rf_NdotV = NdotV;
           rf_dsq = dsq; rf_t1 = t1;
         }
@@ -324,10 +325,10 @@ __device__ __forceinline__ void scatter_surface(Lane& L, const Ctx<ADJ>& c, floa
     float NdotV = tmax(1e-4f, dot(N, V));
     float D = ggx_D(NdotH, sm.r);
     float G = ggx_G(N, V, ndir, sm.r);
-    float bsdf = (D * G * F) / (4.0f * NdotL * NdotV);
-    pdf = (D * NdotH) / (4.0f * ia);
+    float bsdf = fdiv(D * G * F, 4.0f * NdotL * NdotV);
+    pdf = fdiv(D * NdotH, 4.0f * ia);
     float cc = branch == 2 ? fabsf(dot(N, ndir)) : tmax(0.0f, dot(N, ndir));
-    est = (bsdf * cc) / pdf;
+    est = fdiv(bsdf * cc, pdf);
     g_NdotL = NdotL; g_NdotH = NdotH; g_NdotV = NdotV; g_c = cc;
   }
   if (ADJ && L.pass == 1) {
@@ -416,9 +417,9 @@ __device__ __forceinline__ void on_extension(Lane& L, const Ctx<ADJ>& c, const H
   L.n_trav++;
   const int ab = L.active;
   MatRec vm = get_mat(s, L.vol, ab);
-  float sa = (4.0f * SR_PI * vm.k) / s.lam_m[ab];
+  float sa = fdiv(4.0f * SR_PI * vm.k, s.lam_m[ab]);
   const U4 re4 = draws(a, L, L.seg, SR_SLOT_EXT);
-  float dvol = -logf(-re4.u[0] + 1.0f - 1e-7f) / (vm.ss + sa + 1e-6f);
+  float dvol = fdiv(-logf(-re4.u[0] + 1.0f - 1e-7f), vm.ss + sa + 1e-6f);
   float ht = h.t;
   f3 hn = mk(0.f, 0.f, 0.f), hsn = mk(0.f, 0.f, 0.f);
   int hfront = 0, hmat = 0;
@@ -439,7 +440,7 @@ __device__ __forceinline__ void on_extension(Lane& L, const Ctx<ADJ>& c, const H
   L.esn = hsn * ff - hsn * (1.0f - ff);
   L.efront = hfront;
   float T = expf(-(sa + vm.ss) * travel);
-  float emit = ((-T + 1.0f) * vm.em) / (sa + 1e-6f);
+  float emit = fdiv((-T + 1.0f) * vm.em, sa + 1e-6f);
   float atten = T + emit;
   float est = 0.0f;
   float em_new = 0.0f;
@@ -453,7 +454,7 @@ __device__ __forceinline__ void on_extension(Lane& L, const Ctx<ADJ>& c, const H
   } else if (travel >= 0.0f) {
     L.emat = -1;
     float re = re4.u[1];
-    float st = sa / (vm.ss + sa + 1e-6f);
+    float st = fdiv(sa, vm.ss + sa + 1e-6f);
     if (re <= st) {
       absorb_branch = true;
       const int eid = s.mat_eem[L.vol];
@@ -469,15 +470,16 @@ __device__ __forceinline__ void on_extension(Lane& L, const Ctx<ADJ>& c, const H
           float r2 = re4.u[3];
           float cum = 0.0f;
           int nw = ab;
+          const float rn = frcp(norm);
           for (int j = 0; j < s.B; ++j) {
-            cum += row[j] / norm;
+            cum += row[j] * rn;
             if (r2 < cum) { nw = j; break; }
           }
           L.enew = nw;
           const U4 rx = draws(a, L, L.seg, SR_SLOT_AUX);
           L.edir = random_in_unit_sphere(rx.u[2], rx.u[3]);
           em_new = get_mat(s, L.vol, nw).em;
-          float em2 = ((-(1.0f - T) + 1.0f) * em_new) / (sa + 1e-6f);
+          float em2 = fdiv((-(1.0f - T) + 1.0f) * em_new, sa + 1e-6f);
           est = (1.0f - T) + em2;
         } else {
           est = atten;
@@ -536,9 +538,9 @@ __device__ __forceinline__ void on_extension(Lane& L, const Ctx<ADJ>& c, const H
     if (smt.se) {  // direct hit on an emitter (main.py:1550-1559)
       float d2 = travel * travel;
       float cl = tmax(0.0f, dot(hn, -L.dir));
-      float pdfl = (d2 * tmin(1.0f, (float)L.pl)) / (cl * s.light_area[L.active] + 1e-6f);
+      float pdfl = fdiv(d2 * tmin(1.0f, (float)L.pl), cl * s.light_area[L.active] + 1e-6f);
       float den = pdfl * pdfl + L.sp * L.sp + 1e-6f;
-      float w = (L.sp * L.sp) / den;
+      float w = fdiv(L.sp * L.sp, den);
       float contrib = L.tp * smt.em * w;
       if (ADJ && L.pass == 1 && L.sp_dr != 0.0f && isfinite(contrib)) {
         float dlogw = 2.0f / L.sp - 2.0f * L.sp / den;
@@ -592,7 +594,7 @@ __device__ __forceinline__ void on_shadow(Lane& L, const Ctx<ADJ>& c, const HitD
   L.n_trav++;
   const int ab = L.active;
   MatRec vm = get_mat(s, L.svol, ab);
-  float absorb = (4.0f * SR_PI * vm.k) / s.lam_m[ab];
+  float absorb = fdiv(4.0f * SR_PI * vm.k, s.lam_m[ab]);
   const bool sig_span = ADJ && L.svol == c.adj.sig_mat;
   L.stot *= expf(-(absorb + vm.ss) * h.t);
   if (sig_span) L.stot_dsig -= h.t;
@@ -610,7 +612,7 @@ __device__ __forceinline__ void on_shadow(Lane& L, const Ctx<ADJ>& c, const HitD
     f3 inc = sh.sn * ff - sh.sn * (1.0f - ff);
     f3 refr = refract(L.sdir, inc, n_i / n_t);
     float rt = (dot(L.sdir, refr) - 1.0f) * 37.9743f;
-    float af = 1.0f / (1.0f + rt * rt * rt * rt * (3.0f - 2.0f * rt));
+    float af = frcp(1.0f + rt * rt * rt * rt * (3.0f - 2.0f * rt));
     L.stot *= af;
     if (L.stot < 1e-6f || !sm.tv) {
       L.stot = 0.0f;
@@ -631,7 +633,7 @@ __device__ __forceinline__ void on_shadow(Lane& L, const Ctx<ADJ>& c, const HitD
     int lmat = __float_as_int(shl[1].w);
     float cl = tmax(0.0f, dot(ln, -L.sdir));
     float d2 = L.smax * L.smax;
-    float gt = cl / (d2 + 1e-4f);
+    float gt = fdiv(cl, d2 + 1e-4f);
     float bsdf, cosv, pdf;
     if (L.nee_surf != -1) {
       cosv = tmax(0.0f, dot(L.esn, L.sdir));
@@ -643,8 +645,8 @@ __device__ __forceinline__ void on_shadow(Lane& L, const Ctx<ADJ>& c, const HitD
     }
     float factor = bsdf * cosv * L.stot * gt * s.light_area[ab];
     float emis = get_mat(s, lmat, L.band).em;
-    float pdfl = d2 / (tmax(1e-6f, cl) * s.light_area[ab]);
-    float w = (pdfl * pdfl) / (pdfl * pdfl + pdf * pdf + 1e-6f);
+    float pdfl = fdiv(d2, tmax(1e-6f, cl) * s.light_area[ab]);
+    float w = fdiv(pdfl * pdfl, pdfl * pdfl + pdf * pdf + 1e-6f);
     direct = L.tp * emis * factor * w;
     if (ADJ && L.pass == 1 && isfinite(direct) && direct != 0.0f) {
       const AdjMap& m = c.adj;
