@@ -216,11 +216,14 @@ __device__ __forceinline__ void visit_leaf(const float4* __restrict__ tris, int 
 // intersect their leaves together.  This keeps the wave in one of the two
 // loop bodies instead of splitting it between nodes and leaves each step.
 // `nodes` / `tris` are passed by value so they sit in scalar registers for
-// the whole traversal (no reloads inside the loops).
+// the whole traversal (no reloads inside the loops).  Only hits with
+// t < t_init are reported (t_init = 1e9, the reference's miss distance,
+// for a plain closest-hit query); with no such hit the result is
+// {t_init, -1}.
 __device__ __forceinline__ HitD trace_closest(const Node4* __restrict__ nodes, const float4* __restrict__ tris, f3 o,
-                                              f3 d, int* stk, int stride, uint32_t* overflow) {
+                                              f3 d, int* stk, int stride, uint32_t* overflow, float t_init = 1e9f) {
   HitD h;
-  h.t = 1e9f;  // HitRecord(t=1e9), main.py:833
+  h.t = t_init;  // HitRecord(t=1e9), main.py:833
   h.prim = INT32_MIN;
   h.u = 0.f;
   h.v = 0.f;

@@ -812,9 +812,20 @@ __device__ __forceinline__ void path_loop(const PathArgs& P, int* stk, float* ld
     if (L.phase == (do_ext ? PH_EXT : PH_SHD)) {
       f3 ro = do_ext ? L.pos : L.pos + L.sdir * L.soff;
       f3 rd = do_ext ? L.dir : L.sdir;
+      float t_init = 1e9f;
+      if (!do_ext) {
+        // A shadow step through a medium with sigma_a + sigma_s = 0 needs
+        // only the hits before the light: any hit at t >= smax - strav ends
+        // the walk as "reached" with an attenuation of exactly 1 whatever
+        // its t (main.py:1463-1468), so the search starts at that distance.
+        // (Not for the adjoint's sigma_a material: its span derivative uses t.)
+        // (k = 0 and sigma_s = 0 make that sum exactly 0.)
+        const float4 mv = *reinterpret_cast<const float4*>(J.s.mats + L.svol * J.s.B + L.active);
+        if (mv.y == 0.0f && mv.w == 0.0f && !(ADJ && L.svol == P.adj.sig_mat)) t_init = L.smax - L.strav;
+      }
       const Node4* nodes = J.s.nodes;
       const float4* tris = J.s.tris;
-      HitD h = trace_closest(nodes, tris, ro, rd, stk, stride, P.overflow);
+      HitD h = trace_closest(nodes, tris, ro, rd, stk, stride, P.overflow, t_init);
       if (do_ext) on_extension<ADJ>(L, c, h);
       else on_shadow<ADJ>(L, c, h);
     }
