@@ -833,7 +833,11 @@ __device__ __forceinline__ void path_loop(const PathArgs& P, int* stk, float* ld
   }
 }
 
-__global__ __launch_bounds__(256) void sr_path_kernel(PathArgs a) {
+// The traversal is bound by the latency of its dependent node fetches, so
+// the kernel asks for 4 waves per SIMD (128 VGPRs; the compiler spills a
+// few long-lived shading values, none inside the traversal loops).  The
+// 40-entry LDS stack of 256-lane blocks also fits 4 blocks per CU.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void sr_path_kernel(PathArgs a) {
   extern __shared__ int lds_dyn[];
   if (threadIdx.x == 0) atomicMin(a.clocks, wall_clock64());
   path_loop<false>(a, lds_dyn + threadIdx.x, nullptr);
