@@ -2,7 +2,8 @@
 //
 // HBM layout (all arrays 16-byte aligned, one allocation each):
 //   nodes   Node4[n_nodes]      128 B lines: SoA child boxes + child refs
-//   tris    TriRec[n_tris]      48 B: v0|prim, e1, e2 in BVH leaf order
+//   tris    TriRec[n_tris + 1]  48 B: v0|prim, e1, e2 in BVH leaf order, then
+//                               one zero record (triangles are read in pairs)
 //   shade   ShadeRec[n_tris]    64 B: face normal|area, n0|mat, n1, n2,
 //                               indexed by ORIGINAL face id (fetched once
 //                               per closest hit, never during traversal)
