@@ -200,6 +200,66 @@ def bench_trace(n=1 << 22, reps=5, log=print):
     return {"metric": "closest-hit traversal rate, heightfield_1m (1,048,588 tris)", **out}
 
 
+def bench_build(reps=3, log=print):
+    """Scene compile on the host (C++ SAH BVH, sr_upload_scene) against the
+    device (linear BVH on the GPU, sr_upload_scene_gpu) for heightfield_1m,
+    and the path kernel's rate on each tree (one block, same seed)."""
+    import numpy as np
+    import torch
+
+    from spectral import _lib, configs
+    from spectral.render import Renderer
+    from spectral.scene import flatten_scene
+
+    cfg = configs.heightfield_1m()
+    flat = flatten_scene(cfg.scene, cfg.bands, cfg.wl_min, cfg.wl_max)
+    out = {"metric": "scene compile time and path rate by BVH builder, heightfield_1m (1,048,588 tris)"}
+    rs = {}
+    for build in ("host", "device"):
+        t0 = time.time()
+        r = Renderer(flat, cfg.bands, cfg.wl_min, cfg.wl_max, width=cfg.width, height=cfg.height, device="cuda:0",
+                     build=build)
+        torch.cuda.synchronize()
+        rs[build] = (r, time.time() - t0)
+    # device build alone: inputs already resident, best of reps
+    dv = torch.from_numpy(np.ascontiguousarray(flat.tris_v, np.float32)).cuda()
+    dn = torch.from_numpy(np.ascontiguousarray(flat.tris_n, np.float32)).cuda()
+    dm = torch.from_numpy(np.ascontiguousarray(flat.tri_mat, np.int32)).cuda()
+    r = rs["device"][0]
+    best_dev = 1e9
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.time()
+        _lib.call("sr_upload_scene_gpu", r._h, _lib.ptr(dv), _lib.ptr(dn), _lib.ptr(dm), flat.n_tris,
+                  _lib.ptr(flat.mat_f32), _lib.ptr(flat.mat_i32), _lib.ptr(flat.mat_eem), flat.n_mats,
+                  _lib.ptr(flat.eem), flat.n_eem, _lib.ptr(flat.cam))
+        best_dev = min(best_dev, time.time() - t0)
+    best_host = 1e9
+    rh = rs["host"][0]
+    for _ in range(reps):
+        t0 = time.time()
+        rh._upload(flat)
+        best_host = min(best_host, time.time() - t0)
+    for build, (rr, first_s) in rs.items():
+        rr.render(n_blocks=1, max_depth=cfg.depth, seed=cfg.seed)  # warm
+        rr.sync()
+        ms = []
+        for k in range(reps):
+            rr.clear_pixel_space()
+            rr.render(n_blocks=1, max_depth=cfg.depth, seed=cfg.seed + 1 + k)
+            ms.append(rr.last_kernel_ms())
+        st = rr.stats()
+        info = rr.scene_info()
+        out[build] = {"first_upload_s": round(first_s, 3),
+                      "compile_s": round(best_host if build == "host" else best_dev, 4),
+                      "bvh_nodes": info["bvh_nodes"], "bvh4_depth": info["bvh4_depth"],
+                      "kernel_ms_per_block": round(float(np.median(ms)), 3),
+                      "Mseg_per_s": round(st["segments"] / float(np.median(ms)) / 1e3, 1)}
+    out["note"] = ("host compile_s = flattened arrays -> normals, lights, SAH BVH and upload (C++); device "
+                   "compile_s = sr_upload_scene_gpu with the triangles already in HBM")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -227,6 +287,9 @@ def main():
         return
     if args.workload == "trace":
         print(json.dumps(bench_trace()), flush=True)
+        return
+    if args.workload == "build":
+        print(json.dumps(bench_build()), flush=True)
         return
     if args.workload == "batch64":
         print(json.dumps(bench_batch(args.spp_scale, log=lambda m: print(m, file=sys.stderr, flush=True))),

@@ -435,6 +435,70 @@ int sr_update_materials(sr_handle* h, const float* mat_f32, const int* mat_i32, 
   return upload_materials(h, mat_f32, mat_i32, mat_eem, n_mats, eem, n_eem);
 }
 
+// Device scene compile: the same records as sr_upload_scene (main.py:509-631),
+// built on the GPU from device triangle arrays (sr_build.hip).
+int sr_upload_scene_gpu(sr_handle* h, const float* dev_tris_v, const float* dev_tris_n, const int* dev_tri_mat,
+                        int n_tris, const float* mat_f32, const int* mat_i32, const int* mat_eem, int n_mats,
+                        const float* eem, int n_eem, const float* cam) {
+  g_err.clear();
+  if (!h) return fail(SR_EINVAL, "null handle");
+  if (n_tris < 0 || (n_tris > 0 && (!dev_tris_v || !dev_tris_n || !dev_tri_mat)))
+    return fail(SR_EINVAL, "bad triangle arrays");
+  if (n_tris >= (1 << 27)) return fail(SR_EINVAL, "n_tris=%d exceeds 2^27", n_tris);
+  if (!cam) return fail(SR_EINVAL, "cam is NULL");
+  if (n_mats <= 0 || !mat_i32) return fail(SR_EINVAL, "no materials");
+  HIPCHK(hipSetDevice(h->device));
+  int rc = upload_materials(h, mat_f32, mat_i32, mat_eem, n_mats, eem, n_eem);
+  if (rc) return rc;
+  DeviceSceneOut o;
+  char err[256];
+  rc = sr_build_scene_device(dev_tris_v, dev_tris_n, dev_tri_mat, n_tris, mat_i32, n_mats, h->B, h->st, &o, err,
+                             sizeof(err));
+  if (rc) {
+    sr_free_scene_out(&o);
+    return fail(rc, "%s", err);
+  }
+  if (3 * o.depth + 1 > SR_STACK_SIZE) {
+    sr_free_scene_out(&o);
+    return fail(SR_EOVERFLOW, "BVH depth %d exceeds the traversal stack", o.depth);
+  }
+  const int B = h->B;
+  std::vector<float> lm((size_t)B), ln((size_t)B);
+  for (int b = 0; b < B; ++b) {
+    lm[b] = (float)((double)h->wl_min * 1e-9) + (float)b * h->step * 1e-9f;
+    ln[b] = h->wl_min + (float)b * h->step;
+  }
+  if ((rc = h->lam_m.upload(lm.data(), lm.size(), h->st)) || (rc = h->lam_nm.upload(ln.data(), ln.size(), h->st))) {
+    sr_free_scene_out(&o);
+    return rc;
+  }
+  // the handle takes over the device records
+  auto adopt = [](DevBuf& d, void*& p, size_t bytes) {
+    d.release();
+    d.p = p;
+    d.bytes = bytes;
+    p = nullptr;
+  };
+  adopt(h->nodes, o.nodes, o.nodes_bytes);
+  adopt(h->tris, o.tris, o.tris_bytes);
+  adopt(h->shade, o.shade, o.shade_bytes);
+  adopt(h->lights, o.lights, o.lights_bytes);
+  adopt(h->lverts, o.lverts, o.lverts_bytes);
+  adopt(h->light_count, o.light_count, o.light_count_bytes);
+  adopt(h->light_area, o.light_area, o.light_area_bytes);
+  HIPCHK(hipStreamSynchronize(h->st));
+  h->n_tris = n_tris;
+  h->n_nodes = o.n_nodes;
+  h->depth = o.depth;
+  h->max_lights = o.max_lights;
+  h->n_slots = o.n_slots;
+  h->h_light_area = o.h_light_area;
+  h->h_light_count = o.h_light_count;
+  set_camera_host(h, cam);
+  h->has_scene = true;
+  return 0;
+}
+
 int sr_scene_info(sr_handle* h, int* n_nodes, int* bvh_depth, int* max_lights, long long* device_bytes) {
   if (!h) return fail(SR_EINVAL, "null handle");
   if (n_nodes) *n_nodes = h->n_nodes;

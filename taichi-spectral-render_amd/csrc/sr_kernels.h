@@ -1,5 +1,6 @@
 // sr_kernels.h -- kernel argument blocks and host-side launchers.
 #pragma once
+#include <vector>
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 #include <stdint.h>
@@ -8,6 +9,36 @@
 #include "sr_device.h"
 
 #define SR_MAX_PARAMS 1024
+
+// Scene records built on the device by sr_build_scene_device (sr_build.hip);
+// every pointer is a hipMalloc allocation the caller takes over (or frees
+// with sr_free_scene_out).
+struct DeviceSceneOut {
+  void* nodes = nullptr;
+  size_t nodes_bytes = 0;
+  int n_nodes = 0, depth = 0;
+  void* tris = nullptr;
+  size_t tris_bytes = 0;
+  void* shade = nullptr;
+  size_t shade_bytes = 0;
+  void* lights = nullptr;
+  size_t lights_bytes = 0;
+  int max_lights = 0;
+  void* lverts = nullptr;
+  size_t lverts_bytes = 0;
+  int n_slots = 0;
+  void* light_count = nullptr;
+  size_t light_count_bytes = 0;
+  void* light_area = nullptr;
+  size_t light_area_bytes = 0;
+  std::vector<int> h_light_count;
+  std::vector<float> h_light_area;
+};
+// tris_v / tris_n [n][3][3] f32 and tri_mat [n] in device memory; mat_i32
+// [n_mats][2][B] on the host.  Returns 0 or an SR_E* code with `err` set.
+int sr_build_scene_device(const float* tris_v, const float* tris_n, const int* tri_mat, int n, const int* mat_i32,
+                          int n_mats, int B, hipStream_t st, DeviceSceneOut* out, char* err, size_t errlen);
+void sr_free_scene_out(DeviceSceneOut* o);
 
 // Differentiable parameters of the adjoint (sr_render_adjoint): per-band
 // albedo of any material with alb_off[m] >= 0 (param index alb_off[m] + band),

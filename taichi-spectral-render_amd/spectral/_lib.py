@@ -32,6 +32,8 @@ SIGNATURES = {
     "sr_set_stream": ([_vp, _vp], ctypes.c_int),
     "sr_upload_scene": ([_vp, _vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp],
                         ctypes.c_int),
+    "sr_upload_scene_gpu": ([_vp, _vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, _vp, ctypes.c_int,
+                             _vp], ctypes.c_int),
     "sr_set_camera": ([_vp, _vp], ctypes.c_int),
     "sr_update_materials": ([_vp, _vp, _vp, _vp, ctypes.c_int, _vp, ctypes.c_int], ctypes.c_int),
     "sr_scene_info": ([_vp, _c_int_p, _c_int_p, _c_int_p, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),

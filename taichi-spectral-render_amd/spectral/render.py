@@ -29,11 +29,17 @@ def _stream_ptr():
 class Renderer:
     """One scene, one accumulator.
 
-    ``scene`` is a reference-style scene dict or a ``FlatScene``.
+    ``scene`` is a reference-style scene dict or a ``FlatScene``.  ``build``
+    chooses where the scene is compiled (normals, light lists, BVH):
+    ``"host"`` (SAH BVH built in C++ on the CPU, ``sr_upload_scene``) or
+    ``"device"`` (linear BVH built on the GPU, ``sr_upload_scene_gpu``).
     """
 
     def __init__(self, scene, spectral_bands, wavelength_min, wavelength_max, width=None, height=None,
-                 device="cuda"):
+                 device="cuda", build="host"):
+        if build not in ("host", "device"):
+            raise ValueError(f"build must be 'host' or 'device', not {build!r}")
+        self.build = build
         self.bands = int(spectral_bands)
         self.wl_min = float(wavelength_min)
         self.wl_max = float(wavelength_max)
@@ -72,6 +78,14 @@ class Renderer:
 
     def _upload(self, f: FlatScene):
         self._bind_stream()
+        if self.build == "device":
+            dv = torch.from_numpy(np.ascontiguousarray(f.tris_v, np.float32)).to(self.device)
+            dn = torch.from_numpy(np.ascontiguousarray(f.tris_n, np.float32)).to(self.device)
+            dm = torch.from_numpy(np.ascontiguousarray(f.tri_mat, np.int32)).to(self.device)
+            _lib.call("sr_upload_scene_gpu", self._h, _lib.ptr(dv), _lib.ptr(dn), _lib.ptr(dm), f.n_tris,
+                      _lib.ptr(f.mat_f32), _lib.ptr(f.mat_i32), _lib.ptr(f.mat_eem), f.n_mats, _lib.ptr(f.eem),
+                      f.n_eem, _lib.ptr(f.cam))
+            return
         _lib.call("sr_upload_scene", self._h, _lib.ptr(f.tris_v), _lib.ptr(f.tris_n), _lib.ptr(f.tri_mat), f.n_tris,
                   _lib.ptr(f.mat_f32), _lib.ptr(f.mat_i32), _lib.ptr(f.mat_eem), f.n_mats, _lib.ptr(f.eem),
                   f.n_eem, _lib.ptr(f.cam))
