@@ -152,6 +152,17 @@ int sr_render_adjoint(sr_handle* h, int width, int height, uint64_t seed, uint32
 int sr_resolve(sr_handle* h, const float* dev_accum, int width, int height, const float* cmf, const float* xyz2rgb,
                float norm, float exposure, float gamma, int out_w, int out_h, float* dev_rgb);
 
+/* visualize_normals (main.py:1632-1660), the geometry display mode: a
+ * pixel-centre camera ray per render pixel (width x height grid), colour =
+ * shading normal * 0.5 + 0.5, black where the ray misses, nearest-upsampled
+ * into dev_rgb [out_w][out_h][3] f32 as color_rays does. */
+int sr_render_normals(sr_handle* h, int width, int height, int out_w, int out_h, float* dev_rgb);
+
+/* The display conversion of a [n_pixels][3] f32 image in [0, 1] (what the
+ * reference hands to its GUI, main.py:1845) to [n_pixels] RGBA8: clamped to
+ * [0, 1], x * 255 rounded to nearest, alpha 255, NaN shown as 0. */
+int sr_to_rgba8(sr_handle* h, const float* dev_rgb, int64_t n_pixels, void* dev_rgba);
+
 /* Counters of the last sr_render (blocking): path segments (extension
  * traversals, one per executed path-loop iteration), all traversals
  * (extension + shadow), dropped non-finite contributions. */

@@ -691,6 +691,31 @@ int sr_render_adjoint(sr_handle* h, int width, int height, uint64_t seed, uint32
   return 0;
 }
 
+int sr_render_normals(sr_handle* h, int width, int height, int out_w, int out_h, float* dev_rgb) {
+  g_err.clear();
+  if (!h) return fail(SR_EINVAL, "null handle");
+  if (!h->has_scene) return fail(SR_ESTATE, "no scene uploaded");
+  if (!dev_rgb) return fail(SR_EINVAL, "null array");
+  if (width <= 0 || height <= 0 || out_w <= 0 || out_h <= 0) return fail(SR_EINVAL, "bad size");
+  if ((long long)out_w * out_h >= (1LL << 31)) return fail(SR_EINVAL, "display grid too large");
+  HIPCHK(hipSetDevice(h->device));
+  SceneDev s;
+  fill_scene_dev(h, s);
+  if (sr_launch_normals(s, width, height, out_w, out_h, dev_rgb, (uint32_t*)h->overflow.p, h->st))
+    return fail(SR_EHIP, "normals launch failed");
+  return 0;
+}
+
+int sr_to_rgba8(sr_handle* h, const float* dev_rgb, int64_t n_pixels, void* dev_rgba) {
+  g_err.clear();
+  if (!h) return fail(SR_EINVAL, "null handle");
+  if (!dev_rgb || !dev_rgba) return fail(SR_EINVAL, "null array");
+  if (n_pixels < 0) return fail(SR_EINVAL, "bad size");
+  HIPCHK(hipSetDevice(h->device));
+  if (sr_launch_rgba8(dev_rgb, (size_t)n_pixels, dev_rgba, h->st)) return fail(SR_EHIP, "rgba8 launch failed");
+  return 0;
+}
+
 int sr_resolve(sr_handle* h, const float* dev_accum, int width, int height, const float* cmf, const float* xyz2rgb,
                float norm, float exposure, float gamma, int out_w, int out_h, float* dev_rgb) {
   g_err.clear();

@@ -94,3 +94,6 @@ int sr_launch_finalize(float* mean, float* sum, size_t n, int first, int nb, hip
 int sr_launch_fill(float* p, size_t n, float v, hipStream_t st);
 int sr_launch_resolve(const float* acc, int W, int H, int B, const ResolveParams& p, float* out, int OW, int OH,
                       hipStream_t st);
+int sr_launch_normals(const SceneDev& s, int W, int H, int OW, int OH, float* out, uint32_t* overflow,
+                      hipStream_t st);
+int sr_launch_rgba8(const float* rgb, size_t n, void* out, hipStream_t st);

@@ -73,7 +73,66 @@ __global__ void sr_resolve_kernel(const float* __restrict__ acc, int W, int H, i
   o[0] = rgb[0]; o[1] = rgb[1]; o[2] = rgb[2];
 }
 
+// ====================================================================== display
+// visualize_normals, main.py:1632-1660: a pixel-centre camera ray per render
+// pixel, colour = shading normal * 0.5 + 0.5 (black on a miss), then the
+// same nearest upsample to the display grid as color_rays.  One thread per
+// display pixel.
+__global__ __launch_bounds__(256) void sr_normals_kernel(SceneDev s, int W, int H, int OW, int OH,
+                                                         float* __restrict__ out, uint32_t* overflow) {
+  __shared__ int lds_stack[SR_STACK_SIZE * 256];
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= OW * OH) return;
+  const int i = idx / OH, j = idx % OH;
+  const int ri = (int)((float)i / (float)OW * (float)W);
+  const int rj = (int)((float)j / (float)OH * (float)H);
+  const float u = ((float)ri + 0.5f) / (float)W;
+  const float v = ((float)rj + 0.5f) / (float)H;
+  const float aspect = (float)W / (float)H;
+  const f3 cu = mk(s.cam_u[0], s.cam_u[1], s.cam_u[2]);
+  const f3 cv = mk(s.cam_v[0], s.cam_v[1], s.cam_v[2]);
+  const f3 cw = mk(s.cam_w[0], s.cam_w[1], s.cam_w[2]);
+  const f3 d = normalize(cu * (u * 2.0f - 1.0f) * aspect * 1.0f + cv * (v * 2.0f - 1.0f) * 1.0f - cw * 2.0f);
+  const f3 o = mk(s.cam_o[0], s.cam_o[1], s.cam_o[2]);
+  const HitD h = trace_closest(s.nodes, s.tris, o, d, lds_stack + threadIdx.x, blockDim.x, overflow);
+  f3 c = mk(0.0f, 0.0f, 0.0f);
+  if (h.prim >= 0) c = surface_of(s, h, d).sn * 0.5f + mk(0.5f, 0.5f, 0.5f);
+  float* q = out + (size_t)idx * 3;
+  q[0] = c.x;
+  q[1] = c.y;
+  q[2] = c.z;
+}
+
+// Display image [n][3] f32 in [0, 1] -> [n][4] RGBA8 (clamped, rounded to
+// nearest, alpha 255).
+__global__ void sr_rgba8_kernel(const float* __restrict__ rgb, size_t n, uchar4* __restrict__ out) {
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  unsigned char c[3];
+  for (int r = 0; r < 3; ++r) {
+    float x = rgb[k * 3 + r];
+    x = x != x ? 0.0f : fminf(fmaxf(x, 0.0f), 1.0f);  // NaN -> 0
+    c[r] = (unsigned char)(x * 255.0f + 0.5f);
+  }
+  out[k] = make_uchar4(c[0], c[1], c[2], 255);
+}
+
 // ====================================================================== launchers
+int sr_launch_normals(const SceneDev& s, int W, int H, int OW, int OH, float* out, uint32_t* overflow,
+                      hipStream_t st) {
+  const long long n = (long long)OW * OH;
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(sr_normals_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, s, W, H, OW, OH, out,
+                     overflow);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int sr_launch_rgba8(const float* rgb, size_t n, void* out, hipStream_t st) {
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(sr_rgba8_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, rgb, n, (uchar4*)out);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int sr_launch_trace(const SceneDev& s, int n, const float* o, const float* d, float* t, int* prim, float* uv,
                     uint32_t* overflow, hipStream_t st) {
   if (n <= 0) return 0;

@@ -35,6 +35,8 @@ SIGNATURES = {
     "sr_upload_scene_gpu": ([_vp, _vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, _vp, ctypes.c_int,
                              _vp], ctypes.c_int),
     "sr_set_camera": ([_vp, _vp], ctypes.c_int),
+    "sr_render_normals": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
+    "sr_to_rgba8": ([_vp, _vp, ctypes.c_int64, _vp], ctypes.c_int),
     "sr_update_materials": ([_vp, _vp, _vp, _vp, ctypes.c_int, _vp, ctypes.c_int], ctypes.c_int),
     "sr_scene_info": ([_vp, _c_int_p, _c_int_p, _c_int_p, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
     "sr_trace_closest": ([_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, ctypes.c_int], ctypes.c_int),

@@ -156,6 +156,25 @@ class Renderer:
                   norm, float(exposure), float(gamma), ow, oh, _lib.ptr(out))
         return out
 
+    def visualize_normals(self, out_width=None, out_height=None):
+        """visualize_normals (main.py:1632-1660) -> [out_w, out_h, 3] tensor:
+        shading normal * 0.5 + 0.5 of the pixel-centre camera ray, black on a
+        miss, at the render resolution nearest-upsampled to the display."""
+        ow = self.width if out_width is None else int(out_width)
+        oh = self.height if out_height is None else int(out_height)
+        out = torch.empty((ow, oh, 3), dtype=torch.float32, device=self.device)
+        self._bind_stream()
+        _lib.call("sr_render_normals", self._h, self.width, self.height, ow, oh, _lib.ptr(out))
+        return out
+
+    def to_rgba8(self, rgb):
+        """Display conversion of a [w, h, 3] image in [0, 1] -> [w, h, 4] uint8."""
+        rgb = rgb.contiguous()
+        out = torch.empty(rgb.shape[:-1] + (4,), dtype=torch.uint8, device=rgb.device)
+        self._bind_stream()
+        _lib.call("sr_to_rgba8", self._h, _lib.ptr(rgb), int(rgb.numel() // 3), _lib.ptr(out))
+        return out
+
     # ------------------------------------------------------------ measurement / parity
     def stats(self):
         s = ctypes.c_uint64()
