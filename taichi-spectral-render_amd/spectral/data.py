@@ -44,13 +44,123 @@ def _load_table(src):
     return np.asarray(src, dtype=np.float64)
 
 
-class RefractiveIndex:
-    """n(lambda), k(lambda) from a refractiveindex.info YAML file (data.py:6-83)."""
+def _dispersion(kind, c, lam_um):
+    """refractiveindex.info dispersion formula ``kind`` (1-9) with
+    coefficients c = [C1, C2, ...] at wavelengths lam_um (micrometres), in
+    float64; returns n.  The formulas, as the database documents them:
 
-    def __init__(self, file_path, wavelength_min, wavelength_max, spectral_bands):
+    1 Sellmeier     n^2 - 1 = C1 + sum C(2i) L^2 / (L^2 - C(2i+1)^2)
+    2 Sellmeier-2   n^2 - 1 = C1 + sum C(2i) L^2 / (L^2 - C(2i+1))
+    3 Polynomial    n^2 = C1 + sum C(2i) L^C(2i+1)
+    4 RI.INFO       n^2 = C1 + C2 L^C3 / (L^2 - C4^C5) + C6 L^C7 / (L^2 - C8^C9) + sum_{i>=10} C(i) L^C(i+1)
+    5 Cauchy        n = C1 + sum C(2i) L^C(2i+1)
+    6 Gases         n - 1 = C1 + sum C(2i) / (C(2i+1) - L^-2)
+    7 Herzberger    n = C1 + C2 / (L^2 - 0.028) + C3 / (L^2 - 0.028)^2 + C4 L^2 + C5 L^4 + C6 L^6
+    8 Retro         (n^2 - 1) / (n^2 + 2) = C1 + C2 L^2 / (L^2 - C3) + C4 L^2
+    9 Exotic        n^2 = C1 + C2 / (L^2 - C3) + C4 (L - C5) / ((L - C5)^2 + C6)
+    """
+    c = [float(x) for x in c]
+    m = len(c)
+    C = c + [0.0] * 18
+    L = np.asarray(lam_um, dtype=np.float64)
+    L2 = L * L
+    pairs = [(C[i], C[i + 1]) for i in range(1, m - 1, 2)]
+    if kind == 1:
+        return np.sqrt(1.0 + C[0] + sum(a * L2 / (L2 - b * b) for a, b in pairs))
+    if kind == 2:
+        return np.sqrt(1.0 + C[0] + sum(a * L2 / (L2 - b) for a, b in pairs))
+    if kind == 3:
+        return np.sqrt(C[0] + sum(a * L ** b for a, b in pairs))
+    if kind == 4:
+        n2 = C[0] + C[1] * L ** C[2] / (L2 - C[3] ** C[4]) + C[5] * L ** C[6] / (L2 - C[7] ** C[8])
+        n2 = n2 + sum(C[i] * L ** C[i + 1] for i in range(9, max(m - 1, 9), 2))
+        return np.sqrt(n2)
+    if kind == 5:
+        return C[0] + sum(a * L ** b for a, b in pairs)
+    if kind == 6:
+        return 1.0 + C[0] + sum(a / (b - 1.0 / L2) for a, b in pairs)
+    if kind == 7:
+        q = 1.0 / (L2 - 0.028)
+        return C[0] + C[1] * q + C[2] * q * q + C[3] * L2 + C[4] * L2 ** 2 + C[5] * L2 ** 3
+    if kind == 8:
+        r = C[0] + C[1] * L2 / (L2 - C[2]) + C[3] * L2
+        return np.sqrt((1.0 + 2.0 * r) / (1.0 - r))
+    if kind == 9:
+        return np.sqrt(C[0] + C[1] / (L2 - C[2]) + C[3] * (L - C[4]) / ((L - C[4]) ** 2 + C[5]))
+    raise ValueError(f"unknown dispersion formula {kind}")
+
+
+def _parse_columns(raw, ncol):
+    rows = []
+    for line in str(raw).strip().split("\n"):
+        parts = line.split()
+        if len(parts) == ncol:
+            try:
+                rows.append([float(x) for x in parts])
+            except ValueError:
+                print(f"Warning: Could not parse line in YAML data: '{line}'")
+    return np.array(rows, dtype=np.float64).reshape(-1, ncol)
+
+
+class RefractiveIndex:
+    """n(lambda), k(lambda) from a refractiveindex.info YAML file (data.py:6-83).
+
+    With ``formulas=False`` (the default) this is the reference loader: the
+    first ``tabulated nk`` block only, anything else prints an error and
+    leaves ``n``/``k`` as ``None``.  ``formulas=True`` reads every variant of
+    the database format: ``tabulated nk``, ``tabulated n``, ``tabulated k``
+    and ``formula 1`` .. ``formula 9`` (see ``_dispersion``); n comes from
+    the first entry that gives n, k from the first that gives k (0 if none).
+    Tables are interpolated with ``np.interp`` as in the reference; formulas
+    are evaluated at the band wavelengths clamped to the entry's
+    ``wavelength_range`` (the same constant continuation ``np.interp`` gives
+    a table)."""
+
+    def __init__(self, file_path, wavelength_min, wavelength_max, spectral_bands, formulas=False):
         self.file_path = file_path
+        self.formulas = bool(formulas)
         self.target_wavelengths_nm = band_wavelengths(wavelength_min, wavelength_max, spectral_bands)
         self.n, self.k = self._load_and_interpolate_data()
+
+    def _interp(self, wl_um, vals):
+        wl_nm = np.asarray(wl_um) * 1000.0
+        order = np.argsort(wl_nm)
+        return np.interp(self.target_wavelengths_nm, wl_nm[order], np.asarray(vals)[order])
+
+    def _all_variants(self, data):
+        n = k = None
+        for item in (data or {}).get("DATA", []):
+            kind = str(item.get("type", ""))
+            if kind.startswith("formula"):
+                if n is not None:
+                    continue
+                f = int(kind.split()[1])
+                coef = [float(x) for x in str(item.get("coefficients", "")).split()]
+                rng = [float(x) for x in str(item.get("wavelength_range", "0 1e9")).split()]
+                lam = np.clip(self.target_wavelengths_nm / 1000.0, rng[0], rng[1])
+                n = _dispersion(f, coef, lam)
+            elif kind == "tabulated nk":
+                t = _parse_columns(item.get("data", ""), 3)
+                if len(t) == 0:
+                    continue
+                if n is None:
+                    n = self._interp(t[:, 0], t[:, 1])
+                if k is None:
+                    k = self._interp(t[:, 0], t[:, 2])
+            elif kind in ("tabulated n", "tabulated k"):
+                t = _parse_columns(item.get("data", ""), 2)
+                if len(t) == 0:
+                    continue
+                if kind == "tabulated n" and n is None:
+                    n = self._interp(t[:, 0], t[:, 1])
+                if kind == "tabulated k" and k is None:
+                    k = self._interp(t[:, 0], t[:, 1])
+        if n is None:
+            print(f"Error: Could not find refractive index data in {self.file_path}.")
+            return None, None
+        if k is None:
+            k = np.zeros_like(n)
+        return np.asarray(n).tolist(), np.asarray(k).tolist()
 
     def _load_and_interpolate_data(self):
         try:
@@ -65,6 +175,8 @@ class RefractiveIndex:
         except yaml.YAMLError as e:
             print(f"Error parsing YAML file: {e}")
             return None, None
+        if self.formulas:
+            return self._all_variants(data)
 
         raw = ""
         for item in (data or {}).get("DATA", []):
@@ -129,15 +241,18 @@ class ColourSystem:
 
 
 class SpecularVec:
-    """A 1-D spectrum table times ``scale`` (data.py:130-143)."""
+    """A spectrum table times ``scale`` (data.py:130-143).  ``column``
+    picks the curve of a multi-curve table (the spectrum editor exports one
+    column per curve); 1, the reference's only column, by default."""
 
-    def __init__(self, file_path, scale, wavelength_min, wavelength_max, spectral_bands):
+    def __init__(self, file_path, scale, wavelength_min, wavelength_max, spectral_bands, column=1):
         self.raw = _load_table(file_path)
+        self.column = int(column)
         self.target_wavelengths = band_wavelengths(wavelength_min, wavelength_max, spectral_bands)
         self.vec = self._interpolate(scale)
 
     def _interpolate(self, scale):
-        return (np.interp(self.target_wavelengths, self.raw[:, 0], self.raw[:, 1]) * scale).tolist()
+        return (np.interp(self.target_wavelengths, self.raw[:, 0], self.raw[:, self.column]) * scale).tolist()
 
 
 class EEMMatrix:
@@ -167,6 +282,50 @@ class EEMMatrix:
         wl = self.target_wavelengths
         m[wl[None, :] <= wl[:, None]] = 0.0
         return np.maximum(0, m)
+
+
+# ----------------------------------------------------------------------------
+# The curve editors' export formats.
+
+def _js_round(x):  # Math.round: halves round up
+    return int(np.floor(x + 0.5))
+
+
+def spec_table_text(wavelengths, curves):
+    """The spectrum editor's export (SpecEdit.html:433-467): one line per
+    sample, the wavelength rounded to an integer, then one value per curve
+    with 5 decimals.  Readable by ``SpecularVec`` (``column`` = curve + 1) and,
+    with three curves, by ``ColourSystem`` as a CMF table."""
+    wl = np.asarray(wavelengths, dtype=np.float64)
+    cv = np.atleast_2d(np.asarray(curves, dtype=np.float64))
+    if cv.shape[1] != wl.shape[0]:
+        cv = cv.T
+    lines = []
+    for i, w in enumerate(wl):
+        lines.append(" ".join([str(_js_round(w))] + [f"{v:.5f}" for v in cv[:, i]]))
+    return "\n".join(lines) + "\n"
+
+
+def eem_table_text(start, end, excitation, emission, amplitude):
+    """The EEM editor's export (EEMEdit.html:464-505) for n samples between
+    ``start`` and ``end`` nm: a comment line, a header row of 0 and the
+    emission wavelengths, then per excitation wavelength the values
+    excitation[i] * emission[j] * amplitude for j > i and 0 otherwise (6
+    decimals).  Readable by ``EEMMatrix(file_path=...)``."""
+    ex = np.asarray(excitation, dtype=np.float64)
+    em = np.asarray(emission, dtype=np.float64)
+    n = ex.shape[0]
+    if n < 2 or em.shape[0] != n:
+        raise ValueError("excitation and emission need the same length >= 2")
+    step = (end - start) / (n - 1)
+    out = ["# EEM Data: Rows = Excitation, Cols = Emission", " ".join(["0"] + [f"{start + j * step:.2f}"
+                                                                            for j in range(n)])]
+    for i in range(n):
+        row = [f"{start + i * step:.2f}"]
+        for j in range(n):
+            row.append(f"{(ex[i] * em[j] * amplitude if j > i else 0.0):.6f}")
+        out.append(" ".join(row))
+    return "\n".join(out) + "\n"
 
 
 # ----------------------------------------------------------------------------
