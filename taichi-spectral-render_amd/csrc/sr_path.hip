@@ -563,10 +563,16 @@ __device__ __forceinline__ void on_extension(Lane& L, const Ctx<ADJ>& c, const H
   const LightRec* lr = s.lights + (size_t)ab * s.max_lights;
   const U4 rn = draws(a, L, L.seg, SR_SLOT_NEE);
   float r = rn.u[0];
-  int e = 0;
-  for (int i = 0; i < nl; ++i) {
-    if (r < lr[i].cdf) { e = i; break; }
+  // first i with r < cdf[i] (main.py:1310-1313) by binary search over the
+  // non-decreasing CDF: the same light as the linear scan, O(log n) reads
+  // (0 if none, the scan's fallback)
+  int lo = 0, hi = nl;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (r < lr[mid].cdf) hi = mid;
+    else lo = mid + 1;
   }
+  const int e = lo < nl ? lo : 0;
   LightRec le = lr[e];
   const float4* lv = s.lverts + (size_t)le.slot * 3;
   f3 v0 = xyz(lv[0]), v1 = xyz(lv[1]), v2 = xyz(lv[2]);
