@@ -5,10 +5,12 @@
 // infeasible at 1M triangles.  Closest-hit results do not depend on the
 // tree's shape, only on the triangle set, so this builder is free to choose
 // its own: binned SAH (16 bins per axis) into a binary tree with leaves of
-// <= 4 triangles, then a level collapse to 4-wide nodes.  Depth is bounded
-// (SR_MAX_BVH4_DEPTH) by switching to object-median splits when a subtree
-// runs short of depth budget, so the per-lane traversal stack can never
-// overflow.  Child boxes are padded outward so the f32 slab test is
+// <= 4 triangles, then a collapse to 4-wide nodes that fills each node by
+// opening its largest-area inner entry until it has four slots.  Depth is
+// bounded (SR_MAX_BVH4_DEPTH): the binary builder switches to object-median
+// splits when a subtree runs short of depth budget, and a collapse that
+// would come out deeper is redone taking children or grandchildren (half the
+// binary depth), so the per-lane traversal stack can never overflow.  Child boxes are padded outward so the f32 slab test is
 // conservative for every ray that hits a contained triangle.
 #include <algorithm>
 #include <cmath>
@@ -165,6 +167,7 @@ struct Collapse {
   std::vector<Node4>& out;
   std::vector<int>& tri_order;
   int max_depth = 0;
+  bool greedy = true;
 
   Collapse(const std::vector<B2>& b, const std::vector<int>& i, float p, std::vector<Node4>& o, std::vector<int>& t)
       : b2(b), idx(i), pad(p), out(o), tri_order(t) {}
@@ -175,10 +178,34 @@ struct Collapse {
     int slots[4];
     int ns = 0;
     const B2& nd = b2[n];
-    for (int c : {nd.left, nd.right}) {
-      const B2& cn = b2[c];
-      if (cn.count > 0) slots[ns++] = c;
-      else { slots[ns++] = cn.left; slots[ns++] = cn.right; }
+    if (greedy) {
+      // open the inner frontier entry with the largest box area until the
+      // node has four slots (fewer, fuller nodes than taking grandchildren)
+      slots[0] = nd.left;
+      slots[1] = nd.right;
+      ns = 2;
+      while (ns < 4) {
+        int best = -1;
+        float ba = -1.f;
+        for (int q = 0; q < ns; ++q) {
+          const B2& c = b2[slots[q]];
+          if (c.count > 0) continue;
+          const float ar = c.box.area();
+          if (ar > ba) { ba = ar; best = q; }
+        }
+        if (best < 0) break;
+        const B2& c = b2[slots[best]];
+        for (int q = ns; q > best + 1; --q) slots[q] = slots[q - 1];
+        slots[best] = c.left;
+        slots[best + 1] = c.right;
+        ++ns;
+      }
+    } else {  // children, or grandchildren of inner children: depth <= binary depth / 2
+      for (int c : {nd.left, nd.right}) {
+        const B2& cn = b2[c];
+        if (cn.count > 0) slots[ns++] = c;
+        else { slots[ns++] = cn.left; slots[ns++] = cn.right; }
+      }
     }
     int me = (int)out.size();
     out.emplace_back();
@@ -267,6 +294,15 @@ int sr_build_bvh4(const float* v, int n, std::vector<Node4>& nodes, std::vector<
     return 1;
   }
   c.emit(root, 0);
+  if (c.max_depth > SR_MAX_BVH4_DEPTH) {
+    // the greedy collapse can deepen a lopsided tree: take grandchildren,
+    // which keeps the 4-wide depth at half the bounded binary depth
+    nodes.clear();
+    tri_order.clear();
+    c.greedy = false;
+    c.max_depth = 0;
+    c.emit(root, 0);
+  }
   if ((int)tri_order.size() != n) return -1;
   return c.max_depth;
 }
