@@ -364,8 +364,8 @@ int sr_upload_scene(sr_handle* h, const float* tris_v, const float* tris_n, cons
   int depth = sr_build_bvh4(tris_v, n_tris, nodes, order);
   if (depth < 0) return fail(SR_EINVAL, "BVH build failed");
   if (3 * depth + 1 > SR_STACK_SIZE) return fail(SR_EOVERFLOW, "BVH depth %d exceeds the traversal stack", depth);
-  // one zero record past the end: leaves fetch triangles in pairs
-  std::vector<float4> tr((order.size() + 1) * 3, make_float4(0.f, 0.f, 0.f, 0.f));
+  // SR_LEAF_MAX - 1 zero records past the end: a leaf's four records are read at once
+  std::vector<float4> tr((order.size() + SR_LEAF_MAX - 1) * 3, make_float4(0.f, 0.f, 0.f, 0.f));
   for (size_t i = 0; i < order.size(); ++i) {
     int f = order[i];
     const float* p = tris_v + (size_t)f * 9;

@@ -288,8 +288,8 @@ __global__ void k_tris(const unsigned* __restrict__ order, const float* __restri
                        float4* __restrict__ tris) {
 #pragma clang fp contract(off)
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s > n) return;
-  if (s == n) {  // zero record past the end (leaves read triangles in pairs)
+  if (s >= n + SR_LEAF_MAX - 1) return;
+  if (s >= n) {  // zero records past the end (a leaf's four records are read at once)
     tris[(size_t)s * 3 + 0] = tris[(size_t)s * 3 + 1] = tris[(size_t)s * 3 + 2] = make_float4(0.f, 0.f, 0.f, 0.f);
     return;
   }
@@ -569,11 +569,12 @@ int sr_build_scene_device(const float* tris_v, const float* tris_n, const int* t
   out->n_nodes = n_nodes;
   out->depth = depth;
 
-  // triangle records in leaf (sorted) order, plus the zero record
-  out->tris_bytes = ((size_t)n + 1) * 3 * sizeof(float4);
+  // triangle records in leaf (sorted) order, plus SR_LEAF_MAX - 1 zero records
+  out->tris_bytes = ((size_t)n + SR_LEAF_MAX - 1) * 3 * sizeof(float4);
   out->tris = take(out->tris_bytes);
   BALLOC(out->tris);
-  hipLaunchKernelGGL(k_tris, dim3(blocks(n + 1)), dim3(256), 0, st, order, tris_v, n, (float4*)out->tris);
+  hipLaunchKernelGGL(k_tris, dim3(blocks(n + SR_LEAF_MAX - 1)), dim3(256), 0, st, order, tris_v, n,
+                     (float4*)out->tris);
   BCHK(hipGetLastError());
 
   // light lists

@@ -196,20 +196,23 @@ __device__ __forceinline__ void visit_node(const Node4* __restrict__ nodes, int 
   }
 }
 
-// Triangles are fetched two at a time, both before either is tested, so a
-// leaf costs one memory round trip per pair rather than per triangle.  The
-// result does not depend on the test order (ties go to the smaller face id).
+// All SR_LEAF_MAX (4) triangle records of a leaf are fetched before any is
+// tested, so a leaf costs one memory round trip whatever its size (the
+// triangle array ends with SR_LEAF_MAX - 1 zero records, so the reads stay
+// in bounds).  The result does not depend on the test order (ties go to the
+// smaller face id).
+static_assert(SR_LEAF_MAX == 4, "visit_leaf reads four triangle records");
 __device__ __forceinline__ void visit_leaf(const float4* __restrict__ tris, int ref, f3 o, f3 d, HitD& h) {
   const uint32_t r = (uint32_t)ref;
   const int first = (int)((r & ~SR_LEAF_BIT) >> 3);
   const int cnt = (int)(r & 7u);
-  for (int k = 0; k < cnt; k += 2) {
-    const float4* tp = tris + (size_t)(first + k) * 3;
-    // tp[3..5] always exists: the triangle array ends with a padding record
-    const float4 a0 = tp[0], b0 = tp[1], c0 = tp[2], a1 = tp[3], b1 = tp[4], c1 = tp[5];
-    intersect_tri(o, d, a0, b0, c0, h.t, h.prim, h.u, h.v);
-    if (k + 1 < cnt) intersect_tri(o, d, a1, b1, c1, h.t, h.prim, h.u, h.v);
-  }
+  const float4* tp = tris + (size_t)first * 3;
+  const float4 a0 = tp[0], b0 = tp[1], c0 = tp[2], a1 = tp[3], b1 = tp[4], c1 = tp[5];
+  const float4 a2 = tp[6], b2 = tp[7], c2 = tp[8], a3 = tp[9], b3 = tp[10], c3 = tp[11];
+  intersect_tri(o, d, a0, b0, c0, h.t, h.prim, h.u, h.v);
+  if (cnt > 1) intersect_tri(o, d, a1, b1, c1, h.t, h.prim, h.u, h.v);
+  if (cnt > 2) intersect_tri(o, d, a2, b2, c2, h.t, h.prim, h.u, h.v);
+  if (cnt > 3) intersect_tri(o, d, a3, b3, c3, h.t, h.prim, h.u, h.v);
 }
 
 // Closest hit over the 4-wide BVH.  `stk` is this lane's LDS stack column

@@ -2,8 +2,9 @@
 //
 // HBM layout (all arrays 16-byte aligned, one allocation each):
 //   nodes   Node4[n_nodes]      128 B lines: SoA child boxes + child refs
-//   tris    TriRec[n_tris + 1]  48 B: v0|prim, e1, e2 in BVH leaf order, then
-//                               one zero record (triangles are read in pairs)
+//   tris    TriRec[n_tris + 3]  48 B: v0|prim, e1, e2 in BVH leaf order, then
+//                               SR_LEAF_MAX - 1 zero records (a leaf's four
+//                               records are read at once)
 //   shade   ShadeRec[n_tris]    64 B: face normal|area, n0|mat, n1, n2,
 //                               indexed by ORIGINAL face id (fetched once
 //                               per closest hit, never during traversal)
@@ -15,7 +16,7 @@
 
 #define SR_STACK_SIZE 40     // per-lane LDS traversal stack entries
 #define SR_MAX_BVH4_DEPTH 13 // builder guarantee: 3 * depth + 1 <= stack
-#define SR_LEAF_MAX 4        // triangles per leaf
+#define SR_LEAF_MAX 4        // triangles per leaf (the traversal reads 4 records per leaf)
 #define SR_MAX_BANDS 64
 
 // Child reference in Node4::child: >= 0 is an inner node index; with the
