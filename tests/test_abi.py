@@ -38,3 +38,18 @@ def test_version_and_argument_errors():
     assert lib.sr_create(31, 700.0, 400.0, ctypes.byref(h)) == -1
     assert lib.sr_render(None, 8, 8, 0, 0, 0, 1, 4, 0, None) == -1
     assert lib.sr_trace_closest(None, 1, None, None, None, None, None, 0) == -1
+
+
+def test_create_rejects_bad_bands_and_wavelengths():
+    """sr_create validates its arguments before touching the device."""
+    import ctypes
+
+    from spectral import _lib
+
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    for bands, lo, hi in ((1, 400.0, 700.0), (65, 400.0, 700.0), (31, 700.0, 400.0), (31, 0.0, 700.0)):
+        assert lib.sr_create(bands, lo, hi, ctypes.byref(h)) == -1
+        assert h.value is None
+        msg = lib.sr_last_error().decode()
+        assert ("bands" in msg) or ("wl_min" in msg)
