@@ -93,10 +93,12 @@ def adjoint_ratio(reps=2):
                         f"{cfg.spp} spp, depth {cfg.depth}, {ds.n_params} params ({segs} segments per forward)"}
 
 
-def bench_batch(spp_scale=8, graph=True, log=print):
+def bench_batch(spp_scale=8, graph=True, log=print, build="host"):
     """64-scene batch in one persistent launch (hipGraph) against sequential
     per-scene launches of the same jobs.  spp is divided by spp_scale to
-    bound the run time (full spp: ~19G paths)."""
+    bound the run time (full spp: ~19G paths).  ``build`` chooses where the
+    64 scenes are compiled (host or device; the trees are identical); the
+    scene-compile time is reported beside the render figures."""
     import numpy as np
     import torch
 
@@ -105,14 +107,18 @@ def bench_batch(spp_scale=8, graph=True, log=print):
     from spectral.render import Renderer
     from spectral.scene import flatten_scene
 
-    t0 = time.time()
     cfgs = configs.batch64()
+    flats = [flatten_scene(c.scene, c.bands, c.wl_min, c.wl_max) for c in cfgs]
+    torch.cuda.synchronize()
+    t0 = time.time()
     rs = []
-    for c in cfgs:
-        f = flatten_scene(c.scene, c.bands, c.wl_min, c.wl_max)
+    for c, f in zip(cfgs, flats):
         rs.append((c, f.n_tris, Renderer(f, c.bands, c.wl_min, c.wl_max, width=c.width, height=c.height,
-                                         device="cuda:0")))
-    log(f"# batch64: {len(rs)} scenes, {sum(n for _, n, _ in rs)} tris, uploaded in {time.time() - t0:.1f} s")
+                                         device="cuda:0", build=build)))
+    torch.cuda.synchronize()
+    compile_s = time.time() - t0
+    log(f"# batch64: {len(rs)} scenes, {sum(n for _, n, _ in rs)} tris, compiled ({build}) and uploaded in "
+        f"{compile_s:.2f} s")
     spp = [max(1, c.spp // spp_scale) for c, _, _ in rs]
     # autotune on one block per scene
     best, timings = tune([(r, 1, c.depth, c.seed) for c, _, r in rs], blocks=(128, 256), grid_pcts=(100,),
@@ -159,7 +165,9 @@ def bench_batch(spp_scale=8, graph=True, log=print):
             "workload": f"64 scenes (seed 2), spp/{spp_scale}, 31 bands, depth 12, "
                         f"{sum(c.width * c.height * n * c.bands for (c, _, _), n in zip(rs, spp))} paths",
             "segments": segs, "tuned_config": {"block": best[0], "grid_pct": best[1], "chunk": best[2]},
-            "graph": graph, "batched": batched, "sequential": sequential}
+            "graph": graph, "build": build, "scene_compile_s": round(compile_s, 3),
+            "batched": batched,
+            "sequential": sequential}
 
 
 def bench_trace(n=1 << 22, reps=5, log=print):
@@ -202,7 +210,7 @@ def bench_trace(n=1 << 22, reps=5, log=print):
 
 def bench_build(reps=3, log=print):
     """Scene compile on the host (C++ SAH BVH, sr_upload_scene) against the
-    device (linear BVH on the GPU, sr_upload_scene_gpu) for heightfield_1m,
+    device (sr_upload_scene_gpu, the same arrays built on the GPU) for heightfield_1m,
     and the path kernel's rate on each tree (one block, same seed)."""
     import numpy as np
     import torch
@@ -270,6 +278,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--spp-scale", type=int, default=8, help="batch64: divide each scene's spp by this")
+    ap.add_argument("--build", choices=("host", "device"), default="host",
+                    help="batch64: where the scenes are compiled")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration)")
@@ -292,7 +302,8 @@ def main():
         print(json.dumps(bench_build()), flush=True)
         return
     if args.workload == "batch64":
-        print(json.dumps(bench_batch(args.spp_scale, log=lambda m: print(m, file=sys.stderr, flush=True))),
+        print(json.dumps(bench_batch(args.spp_scale, log=lambda m: print(m, file=sys.stderr, flush=True),
+                                     build=args.build)),
               flush=True)
         return
     if args.gpus != 1:

@@ -88,11 +88,10 @@ int sr_upload_scene(sr_handle* h, const float* tris_v, const float* tris_n, cons
  * the GPU: dev_tris_v / dev_tris_n [n_tris][3][3] f32 and dev_tri_mat
  * [n_tris] i32 are device pointers on the handle's device (e.g. torch
  * tensors); materials, EEM and camera are host arrays as for
- * sr_upload_scene.  Face normals/areas and the per-band light lists are
- * identical to the host build; the BVH is a linear BVH over 63-bit Morton
- * codes collapsed to 4-wide, a different tree with the same closest hits
- * (about 28x faster to build than the host SAH tree for 1M faces, about
- * 20% slower to trace).
+ * sr_upload_scene.  Every compiled array -- face records, light lists, the
+ * 4-wide SAH BVH and the leaf-order triangle records -- is byte-identical
+ * to the host build (sr_scene_array reads them back); about 27x faster
+ * than the host build for 1M faces.
  * SR_EINVAL for a material id outside [0, n_mats), SR_EOVERFLOW if the tree
  * is deeper than the traversal stack allows (sr_upload_scene then). */
 int sr_upload_scene_gpu(sr_handle* h, const float* dev_tris_v, const float* dev_tris_n, const int* dev_tri_mat,
@@ -109,6 +108,21 @@ int sr_update_materials(sr_handle* h, const float* mat_f32, const int* mat_i32, 
 /* Scene statistics: BVH node count, 4-wide depth, max lights per band,
  * device bytes held by the scene. */
 int sr_scene_info(sr_handle* h, int* n_nodes, int* bvh_depth, int* max_lights, long long* device_bytes);
+
+/* Copies one compiled scene array to host memory, as the reference's
+ * fields can be read back (triangles, bvh_nodes and the light tables,
+ * main.py:1700-1709; Taichi's field.to_numpy()).  which: SR_ARRAY_*.
+ * *bytes receives the array's size; with dst NULL only the size is
+ * returned.  SR_EINVAL if capacity < size, SR_ESTATE without a scene.
+ * Blocks until the copy is done. */
+#define SR_ARRAY_NODES 0       /* Node4[n_nodes] */
+#define SR_ARRAY_TRIS 1        /* TriRec[n_tris + 3], leaf order */
+#define SR_ARRAY_SHADE 2       /* 4 x float4 per face, face order */
+#define SR_ARRAY_LIGHTS 3      /* LightRec[bands][max(max_lights, 1)] */
+#define SR_ARRAY_LVERTS 4      /* 3 x float4 per light slot */
+#define SR_ARRAY_LIGHT_COUNT 5 /* int[bands] */
+#define SR_ARRAY_LIGHT_AREA 6  /* float[bands] */
+int sr_scene_array(sr_handle* h, int which, void* dst, size_t capacity, size_t* bytes);
 
 /* Closest hit for n rays: trace (main.py:831-864) as a batch.  out_t = 1e9
  * and out_prim = -1 on a miss; out_prim is the ORIGINAL face id (the order

@@ -10,8 +10,14 @@
 // bounded (SR_MAX_BVH4_DEPTH): the binary builder switches to object-median
 // splits when a subtree runs short of depth budget, and a collapse that
 // would come out deeper is redone taking children or grandchildren (half the
-// binary depth), so the per-lane traversal stack can never overflow.  Child boxes are padded outward so the f32 slab test is
-// conservative for every ray that hits a contained triangle.
+// binary depth), so the per-lane traversal stack can never overflow.  Child
+// boxes are padded outward so the f32 slab test is conservative for every ray
+// that hits a contained triangle.
+//
+// Every step is deterministic -- stable partitions, a stable sort for the
+// median split, first-minimum ties in the SAH scan and the collapse -- so
+// the device builder (sr_build.hip) reproduces this tree, node array and
+// triangle order byte for byte.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -130,7 +136,7 @@ struct Build2 {
         const int a = best_axis;
         const float ext = cb.hi[a] - cb.lo[a];
         const float scale = NB / ext;
-        auto it = std::partition(idx.begin() + lo, idx.begin() + hi, [&](int t) {
+        auto it = std::stable_partition(idx.begin() + lo, idx.begin() + hi, [&](int t) {
           int b = (int)((cen[(size_t)t * 3 + a] - cb.lo[a]) * scale);
           b = std::min(std::max(b, 0), NB - 1);
           return b <= best_bin;
@@ -147,7 +153,7 @@ struct Build2 {
         if (ek > e) { e = ek; a = k; }
       }
       mid = lo + n / 2;
-      std::nth_element(idx.begin() + lo, idx.begin() + mid, idx.begin() + hi,
+      std::stable_sort(idx.begin() + lo, idx.begin() + hi,
                        [&](int x, int y) { return cen[(size_t)x * 3 + a] < cen[(size_t)y * 3 + a]; });
     }
     int l = build(lo, mid, depth + 1);

@@ -514,6 +514,35 @@ int sr_scene_info(sr_handle* h, int* n_nodes, int* bvh_depth, int* max_lights, l
   return 0;
 }
 
+// Read-back of the compiled records (the reference's scene fields,
+// main.py:1700-1709); sizes are the logical ones (a buffer may be larger).
+int sr_scene_array(sr_handle* h, int which, void* dst, size_t capacity, size_t* bytes) {
+  g_err.clear();
+  if (!h) return fail(SR_EINVAL, "null handle");
+  if (!h->has_scene) return fail(SR_ESTATE, "no scene uploaded");
+  const size_t B = (size_t)h->B;
+  const DevBuf* b = nullptr;
+  size_t n = 0;
+  switch (which) {
+    case SR_ARRAY_NODES: b = &h->nodes; n = (size_t)h->n_nodes * sizeof(Node4); break;
+    case SR_ARRAY_TRIS: b = &h->tris; n = ((size_t)h->n_tris + SR_LEAF_MAX - 1) * 3 * sizeof(float4); break;
+    case SR_ARRAY_SHADE: b = &h->shade; n = (size_t)std::max(h->n_tris, 1) * 4 * sizeof(float4); break;
+    case SR_ARRAY_LIGHTS: b = &h->lights; n = B * (size_t)std::max(h->max_lights, 1) * sizeof(LightRec); break;
+    case SR_ARRAY_LVERTS: b = &h->lverts; n = (size_t)std::max(h->n_slots, 1) * 3 * sizeof(float4); break;
+    case SR_ARRAY_LIGHT_COUNT: b = &h->light_count; n = B * sizeof(int); break;
+    case SR_ARRAY_LIGHT_AREA: b = &h->light_area; n = B * sizeof(float); break;
+    default: return fail(SR_EINVAL, "unknown scene array %d", which);
+  }
+  if (n > b->bytes) return fail(SR_ESTATE, "scene array %d holds %zu bytes, expected %zu", which, b->bytes, n);
+  if (bytes) *bytes = n;
+  if (!dst) return 0;
+  if (capacity < n) return fail(SR_EINVAL, "capacity %zu < %zu bytes", capacity, n);
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipMemcpyAsync(dst, b->p, n, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  return 0;
+}
+
 int sr_trace_closest(sr_handle* h, int n, const float* orig, const float* dir, float* out_t, int* out_prim,
                      float* out_uv, int flags) {
   g_err.clear();

@@ -39,6 +39,7 @@ SIGNATURES = {
     "sr_to_rgba8": ([_vp, _vp, ctypes.c_int64, _vp], ctypes.c_int),
     "sr_update_materials": ([_vp, _vp, _vp, _vp, ctypes.c_int, _vp, ctypes.c_int], ctypes.c_int),
     "sr_scene_info": ([_vp, _c_int_p, _c_int_p, _c_int_p, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
+    "sr_scene_array": ([_vp, ctypes.c_int, _vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
     "sr_trace_closest": ([_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, ctypes.c_int], ctypes.c_int),
     "sr_render": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int, ctypes.c_int,
                    ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),

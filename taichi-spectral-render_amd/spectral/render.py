@@ -32,7 +32,7 @@ class Renderer:
     ``scene`` is a reference-style scene dict or a ``FlatScene``.  ``build``
     chooses where the scene is compiled (normals, light lists, BVH):
     ``"host"`` (SAH BVH built in C++ on the CPU, ``sr_upload_scene``) or
-    ``"device"`` (linear BVH built on the GPU, ``sr_upload_scene_gpu``).
+    ``"device"`` (the same records built on the GPU, ``sr_upload_scene_gpu``).
     """
 
     def __init__(self, scene, spectral_bands, wavelength_min, wavelength_max, width=None, height=None,
@@ -110,6 +110,19 @@ class Renderer:
         _lib.call("sr_scene_info", self._h, ctypes.byref(n), ctypes.byref(d), ctypes.byref(m), ctypes.byref(b))
         return {"bvh_nodes": n.value, "bvh4_depth": d.value, "max_lights_per_band": m.value,
                 "device_bytes": b.value}
+
+    SCENE_ARRAYS = {"nodes": 0, "tris": 1, "shade": 2, "lights": 3, "lverts": 4, "light_count": 5, "light_area": 6}
+
+    def scene_array(self, name):
+        """The compiled scene array `name` (SCENE_ARRAYS) as raw bytes
+        (numpy uint8), read back from the device."""
+        which = self.SCENE_ARRAYS[name]
+        size = ctypes.c_size_t()
+        _lib.call("sr_scene_array", self._h, which, None, 0, ctypes.byref(size))
+        out = np.empty(size.value, dtype=np.uint8)
+        _lib.call("sr_scene_array", self._h, which, out.ctypes.data_as(ctypes.c_void_p), out.nbytes,
+                  ctypes.byref(size))
+        return out
 
     # ------------------------------------------------------------ reference protocol
     def set_resolution(self, width, height):
