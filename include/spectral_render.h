@@ -122,6 +122,9 @@ int sr_scene_info(sr_handle* h, int* n_nodes, int* bvh_depth, int* max_lights, l
 #define SR_ARRAY_LVERTS 4      /* 3 x float4 per light slot */
 #define SR_ARRAY_LIGHT_COUNT 5 /* int[bands] */
 #define SR_ARRAY_LIGHT_AREA 6  /* float[bands] */
+#define SR_ARRAY_LIGHT_TREE 7  /* LightNode[light tree nodes] (64 B each) */
+#define SR_ARRAY_LIGHT_POWER 8 /* float[light tree nodes][bands] */
+#define SR_ARRAY_LIGHT_KEYS 9  /* {face, leaf}[emissive faces], sorted by face */
 int sr_scene_array(sr_handle* h, int which, void* dst, size_t capacity, size_t* bytes);
 
 /* Closest hit for n rays: trace (main.py:831-864) as a batch.  out_t = 1e9
@@ -138,7 +141,11 @@ int sr_trace_closest(sr_handle* h, int n, const float* orig, const float* dir, f
  * the blocks into the running mean dev_accum [W][H][B] exactly as
  * average_and_reset_block (main.py:1592-1607) does once per block.
  * Randomness: Philox-4x32-10 keyed by seed, counter (i*H + j, block,
- * band | stream << 8, call/4).  flags: reserved, pass 0. */
+ * band | stream << 8, call/4).  flags: 0, or SR_RENDER_LIGHT_TREE. */
+#define SR_RENDER_LIGHT_TREE 1 /* NEE picks its light through the light tree (importance by distance and
+                                  orientation; the TODO of main.py:541) instead of the area CDF
+                                  (main.py:1306-1313): same expected image, lower variance with many
+                                  lights.  Default 0 reproduces the reference's choice exactly. */
 int sr_render(sr_handle* h, int width, int height, uint64_t seed, uint32_t stream, int first_block, int n_blocks,
               int max_depth, int flags, float* dev_accum);
 

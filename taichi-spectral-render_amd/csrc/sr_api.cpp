@@ -13,6 +13,8 @@
 #include "sr_kernels.h"
 
 int sr_build_bvh4(const float* v, int n, std::vector<Node4>& nodes, std::vector<int>& tri_order);
+int sr_build_light_tree(int n, const float* lverts, const int* slots, const int* slot_mat, const unsigned char* se,
+                        int B, std::vector<LightNode>& nodes, std::vector<float>& power, std::vector<LightKey>& keys);
 
 namespace {
 
@@ -82,7 +84,9 @@ struct sr_handle {
   // scene
   bool has_scene = false;
   int n_tris = 0, n_mats = 0, n_eem = 0, n_nodes = 0, depth = 0, max_lights = 0, n_slots = 0;
+  int n_lnodes = 0, n_lkeys = 0;
   DevBuf nodes, tris, shade, mats, mat_eem, eem, lights, lverts, light_count, light_area, lam_m, lam_nm;
+  DevBuf lnodes, lpower, lkeys;  // light tree
   std::vector<float> h_light_area;
   std::vector<int> h_light_count;
   float cam_o[3], cam_u[3], cam_v[3], cam_w[3];
@@ -105,6 +109,10 @@ static void fill_scene_dev(const sr_handle* h, SceneDev& s) {
   s.light_area = (const float*)h->light_area.p;
   s.lam_m = (const float*)h->lam_m.p;
   s.lam_nm = (const float*)h->lam_nm.p;
+  s.lnodes = h->n_lnodes > 0 ? (const LightNode*)h->lnodes.p : nullptr;
+  s.lpower = h->n_lnodes > 0 ? (const float*)h->lpower.p : nullptr;
+  s.lkeys = h->n_lkeys > 0 ? (const LightKey*)h->lkeys.p : nullptr;
+  s.n_lkeys = h->n_lkeys;
   s.B = h->B;
   s.max_lights = std::max(h->max_lights, 1);
   s.M = h->n_mats;
@@ -261,8 +269,8 @@ int sr_destroy(sr_handle* h) {
   (void)hipSetDevice(h->device);
   if (h->st) (void)hipStreamSynchronize(h->st);
   DevBuf* bufs[] = {&h->nodes, &h->tris, &h->shade, &h->mats, &h->mat_eem, &h->eem, &h->lights, &h->lverts,
-                    &h->light_count, &h->light_area, &h->lam_m, &h->lam_nm, &h->sum, &h->work, &h->jobs,
-                    &h->overflow, &h->tmp, &h->alb_off};
+                    &h->light_count, &h->light_area, &h->lam_m, &h->lam_nm, &h->lnodes, &h->lpower, &h->lkeys,
+                    &h->sum, &h->work, &h->jobs, &h->overflow, &h->tmp, &h->alb_off};
   for (DevBuf* b : bufs) b->release();
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -274,6 +282,27 @@ int sr_destroy(sr_handle* h) {
 int sr_set_stream(sr_handle* h, void* stream) {
   if (!h) return fail(SR_EINVAL, "null handle");
   h->st = stream ? (hipStream_t)stream : h->own;
+  return 0;
+}
+
+// Light tree over the light slots (light_tree.cpp), the same for both scene
+// builds: lv [n_slots][3][4] f32, slot faces and their materials.
+static int upload_light_tree(sr_handle* h, int n_slots, const float* lv, const int* slots, const int* slot_mat,
+                             const int* mat_i32, int n_mats, hipStream_t st) {
+  const int B = h->B;
+  std::vector<unsigned char> se((size_t)n_mats * B);
+  for (int m = 0; m < n_mats; ++m)
+    for (int b = 0; b < B; ++b) se[(size_t)m * B + b] = mat_i32[((size_t)m * 2 + 1) * B + b] ? 1 : 0;
+  std::vector<LightNode> ln;
+  std::vector<float> lp;
+  std::vector<LightKey> lk;
+  sr_build_light_tree(n_slots, lv, slots, slot_mat, se.data(), B, ln, lp, lk);
+  int rc;
+  if ((rc = h->lnodes.upload(ln.data(), ln.size(), st))) return rc;
+  if ((rc = h->lpower.upload(lp.data(), lp.size(), st))) return rc;
+  if ((rc = h->lkeys.upload(lk.data(), lk.size(), st))) return rc;
+  h->n_lnodes = (int)ln.size();
+  h->n_lkeys = (int)lk.size();
   return 0;
 }
 
@@ -401,6 +430,13 @@ int sr_upload_scene(sr_handle* h, const float* tris_v, const float* tris_n, cons
   if ((rc = h->shade.upload(shade.data(), shade.size(), st))) return rc;
   if ((rc = h->lights.upload(lr.data(), lr.size(), st))) return rc;
   if ((rc = h->lverts.upload(lv.data(), lv.size(), st))) return rc;
+  {
+    std::vector<int> smat(slots.size());
+    for (size_t s = 0; s < slots.size(); ++s) smat[s] = tri_mat[slots[s]];
+    if ((rc = upload_light_tree(h, (int)slots.size(), (const float*)lv.data(), slots.data(), smat.data(), mat_i32,
+                                n_mats, st)))
+      return rc;
+  }
   if ((rc = h->light_count.upload(lcount.data(), lcount.size(), st))) return rc;
   if ((rc = h->light_area.upload(larea.data(), larea.size(), st))) return rc;
   if ((rc = h->lam_m.upload(lm.data(), lm.size(), st))) return rc;
@@ -486,6 +522,9 @@ int sr_upload_scene_gpu(sr_handle* h, const float* dev_tris_v, const float* dev_
   adopt(h->lverts, o.lverts, o.lverts_bytes);
   adopt(h->light_count, o.light_count, o.light_count_bytes);
   adopt(h->light_area, o.light_area, o.light_area_bytes);
+  if ((rc = upload_light_tree(h, o.n_slots, o.h_lverts.data(), o.h_slots.data(), o.h_slot_mat.data(), mat_i32, n_mats,
+                              h->st)))
+    return rc;
   HIPCHK(hipStreamSynchronize(h->st));
   h->n_tris = n_tris;
   h->n_nodes = o.n_nodes;
@@ -531,11 +570,14 @@ int sr_scene_array(sr_handle* h, int which, void* dst, size_t capacity, size_t* 
     case SR_ARRAY_LVERTS: b = &h->lverts; n = (size_t)std::max(h->n_slots, 1) * 3 * sizeof(float4); break;
     case SR_ARRAY_LIGHT_COUNT: b = &h->light_count; n = B * sizeof(int); break;
     case SR_ARRAY_LIGHT_AREA: b = &h->light_area; n = B * sizeof(float); break;
+    case SR_ARRAY_LIGHT_TREE: b = &h->lnodes; n = (size_t)h->n_lnodes * sizeof(LightNode); break;
+    case SR_ARRAY_LIGHT_POWER: b = &h->lpower; n = (size_t)h->n_lnodes * B * sizeof(float); break;
+    case SR_ARRAY_LIGHT_KEYS: b = &h->lkeys; n = (size_t)h->n_lkeys * sizeof(LightKey); break;
     default: return fail(SR_EINVAL, "unknown scene array %d", which);
   }
   if (n > b->bytes) return fail(SR_ESTATE, "scene array %d holds %zu bytes, expected %zu", which, b->bytes, n);
   if (bytes) *bytes = n;
-  if (!dst) return 0;
+  if (!dst || n == 0) return 0;
   if (capacity < n) return fail(SR_EINVAL, "capacity %zu < %zu bytes", capacity, n);
   HIPCHK(hipSetDevice(h->device));
   HIPCHK(hipMemcpyAsync(dst, b->p, n, hipMemcpyDeviceToHost, h->st));
@@ -626,11 +668,12 @@ static int ensure_sum(sr_handle* h, int width, int height, hipStream_t st) {
 }
 
 static int prepare_path_args(sr_handle* h, PathArgs& a, int width, int height, uint64_t seed, uint32_t stream,
-                             int first_block, int n_blocks, int max_depth) {
+                             int first_block, int n_blocks, int max_depth, int nee_tree = 0) {
   hipStream_t st = h->st;
   int rc = ensure_sum(h, width, height, st);
   if (rc) return rc;
   fill_job(h, h->job_host, width, height, seed, stream, first_block, n_blocks, max_depth);
+  h->job_host.nee_tree = (nee_tree && h->n_lnodes > 0) ? 1 : 0;
   HIPCHK(hipMemcpyAsync(h->jobs.p, &h->job_host, sizeof(Job), hipMemcpyHostToDevice, st));
   if (reset_work(h->work.p, 1, st)) return fail(SR_EHIP, "reset launch failed");
   HIPCHK(hipMemsetAsync(h->overflow.p, 0, 4, st));
@@ -652,14 +695,16 @@ static int prepare_path_args(sr_handle* h, PathArgs& a, int width, int height, u
 int sr_render(sr_handle* h, int width, int height, uint64_t seed, uint32_t stream, int first_block, int n_blocks,
               int max_depth, int flags, float* dev_accum) {
   g_err.clear();
-  (void)flags;
+  if (flags & ~SR_RENDER_LIGHT_TREE) return fail(SR_EINVAL, "unknown render flags 0x%x", flags);
   int rc = check_render_args(h, width, height, first_block, n_blocks, max_depth, stream);
   if (rc) return rc;
   if (!dev_accum) return fail(SR_EINVAL, "dev_accum is NULL");
   HIPCHK(hipSetDevice(h->device));
   if (n_blocks == 0) return 0;
   PathArgs a;
-  if ((rc = prepare_path_args(h, a, width, height, seed, stream, first_block, n_blocks, max_depth))) return rc;
+  if ((rc = prepare_path_args(h, a, width, height, seed, stream, first_block, n_blocks, max_depth,
+                              (flags & SR_RENDER_LIGHT_TREE) ? 1 : 0)))
+    return rc;
   hipStream_t st = h->st;
   int grid = h->grid;
   const unsigned long long waves_needed = ((unsigned long long)h->job_host.n_items + 255) / 256;

@@ -886,6 +886,11 @@ __global__ void k_lverts(const int* __restrict__ slots, int ns, const float* __r
   for (int k = 0; k < 3; ++k) lv[(size_t)s * 3 + k] = make_float4(p[3 * k], p[3 * k + 1], p[3 * k + 2], 0.f);
 }
 
+__global__ void k_slot_mat(const int* __restrict__ slots, int ns, const int* __restrict__ mat, int* __restrict__ out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < ns) out[s] = mat[slots[s]];
+}
+
 struct Tmp {  // build-lifetime device allocation
   void* p = nullptr;
   ~Tmp() { reset(); }
@@ -1218,6 +1223,22 @@ int sr_build_scene_device(const float* tris_v, const float* tris_n, const int* t
   BCHK(hipMemsetAsync(out->lverts, 0, out->lverts_bytes, st));
   if (ns > 0) hipLaunchKernelGGL(k_lverts, dim3(blocks(ns)), dim3(256), 0, st, slots, ns, tris_v, (float4*)out->lverts);
   BCHK(hipGetLastError());
+  // slot faces, their materials and vertices for the light tree (built on the host)
+  out->h_slots.assign((size_t)ns, 0);
+  out->h_slot_mat.assign((size_t)ns, 0);
+  out->h_lverts.assign((size_t)ns * 12, 0.0f);
+  if (ns > 0) {
+    Tmp t_sm;
+    int* smat = t_sm.get<int>(ns);
+    BALLOC(smat);
+    hipLaunchKernelGGL(k_slot_mat, dim3(blocks(ns)), dim3(256), 0, st, slots, ns, tri_mat, smat);
+    BCHK(hipGetLastError());
+    BCHK(hipMemcpyAsync(out->h_slots.data(), slots, (size_t)ns * sizeof(int), hipMemcpyDeviceToHost, st));
+    BCHK(hipMemcpyAsync(out->h_slot_mat.data(), smat, (size_t)ns * sizeof(int), hipMemcpyDeviceToHost, st));
+    BCHK(hipMemcpyAsync(out->h_lverts.data(), out->lverts, (size_t)ns * 12 * sizeof(float), hipMemcpyDeviceToHost,
+                        st));
+    BCHK(hipStreamSynchronize(st));
+  }
   BCHK(hipStreamSynchronize(st));
   return 0;
 }

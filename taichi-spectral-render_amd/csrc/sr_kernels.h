@@ -33,6 +33,9 @@ struct DeviceSceneOut {
   size_t light_area_bytes = 0;
   std::vector<int> h_light_count;
   std::vector<float> h_light_area;
+  // host copies for the light tree build: slot faces, their materials, lverts
+  std::vector<int> h_slots, h_slot_mat;
+  std::vector<float> h_lverts;  // [n_slots][3][4]
 };
 // tris_v / tris_n [n][3][3] f32 and tri_mat [n] in device memory; mat_i32
 // [n_mats][2][B] on the host.  Returns 0 or an SR_E* code with `err` set.
@@ -61,6 +64,7 @@ struct Job {
   uint32_t n_items;  // W * H * n_blocks
   int W, H, depth, first_block, n_blocks, tiled;
   uint32_t seed_lo, seed_hi, stream;
+  int nee_tree;      // 1: NEE picks lights through the light tree (SR_RENDER_LIGHT_TREE)
 };
 
 struct PathArgs {

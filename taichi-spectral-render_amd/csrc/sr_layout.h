@@ -46,6 +46,27 @@ struct alignas(16) LightRec {
   int pad;
 };
 
+// Light tree node (many-light NEE, SR_RENDER_LIGHT_TREE): bounding box and
+// normal cone (axis, half angle theta_o, rounded up) of the emitters below;
+// inner: left/right children; leaf: left = ~slot, right = -1, face = its
+// original face id.  Per-band emitting area in a separate [nodes][B] array.
+struct alignas(16) LightNode {
+  float lo[3];
+  float theta_o;
+  float hi[3];
+  int left;
+  float axis[3];
+  int right;
+  int parent;  // -1 at the root
+  int face;    // leaf: original face id; inner: -1
+  int pad[2];
+};
+static_assert(sizeof(LightNode) == 64, "LightNode is 64 bytes");
+
+struct LightKey {  // emissive face -> its leaf, sorted by face
+  int face, leaf;
+};
+
 static inline int sr_leaf_ref(uint32_t first, uint32_t count) {
   return (int)(SR_LEAF_BIT | (first << 3) | count);
 }

@@ -37,11 +37,16 @@ struct Lane {
   // pending extension event (scatter_volume result)
   f3 en, esn, edir;
   int efront, emat, eaction, enew, nee_surf;
-  // pending shadow walk (shadow_transmittance state)
+  // pending shadow walk (shadow_transmittance state); once the walk is
+  // over, sdir keeps the NEE vertex (the light tree's MIS pdf at the next
+  // emitter hit is taken there)
   f3 sdir;
   float smax, strav, stot, soff;
   int sprim, svol, sstep;
-  float fdirect;  // NEE contribution awaiting finish_segment (PH_FIN)
+  // during the shadow walk (PH_SHD): the NEE light's area over its pick
+  // probability (the band's emitting area for the CDF pick); then the NEE
+  // contribution awaiting finish_segment (PH_FIN)
+  float fdirect;
   uint32_t n_seg, n_trav, n_nf;
   // adjoint only
   int pass;
@@ -407,6 +412,95 @@ __device__ __forceinline__ void finish_segment(Lane& L, const Ctx<ADJ>& c, float
   }
 }
 
+// ------------------------------------------------------------------ light tree
+// Many-light NEE (SR_RENDER_LIGHT_TREE).  A node's importance for a shading
+// point x is its emitting area in the band times a conservative bound of
+// the emitters' cosine towards x over the squared distance: the angle
+// between the node's normal cone and the direction to x, reduced by the
+// cone half angle, the angle the bounding sphere subtends and a margin for
+// f32 rounding, is clamped at 0; an importance of 0 means every emitter
+// below faces away from x (cos_theta_light <= 0, main.py:1333).  The pick
+// probability p of a light then replaces its area fraction: the estimator
+// and the MIS pdfs use A_light / p where the reference uses the band's
+// total emitting area (main.py:1351, 1355, 1556), which is the same thing
+// for the area-CDF pick.
+constexpr float kLtMargin = 2e-3f;
+
+__device__ __forceinline__ float lt_importance(const SceneDev& s, int node, int band, f3 x) {
+  const float P = s.lpower[(size_t)node * s.B + band];
+  if (!(P > 0.0f)) return 0.0f;
+  const LightNode& nd = s.lnodes[node];
+  const f3 lo = mk(nd.lo[0], nd.lo[1], nd.lo[2]), hi = mk(nd.hi[0], nd.hi[1], nd.hi[2]);
+  const f3 dv = x - (lo + hi) * 0.5f;
+  const f3 ext = hi - lo;
+  const float r2 = 0.25f * dot(ext, ext);
+  const float d2 = dot(dv, dv);
+  float cb = 1.0f;  // inside the bounding sphere: no angular bound
+  if (d2 > r2) {
+    const float d = sqrtf(d2);
+    const float ct = dot(mk(nd.axis[0], nd.axis[1], nd.axis[2]), dv) / d;
+    const float th = acosf(fminf(1.0f, fmaxf(-1.0f, ct)));
+    const float thu = asinf(fminf(1.0f, sqrtf(r2 / d2)));
+    const float thp = fmaxf(0.0f, th - nd.theta_o - thu - kLtMargin);
+    if (thp >= 0.5f * SR_PI) return 0.0f;
+    cb = cosf(thp);
+  }
+  // distance to the centre, floored for points near the centre of a large node
+  return P * cb / fmaxf(d2, 1e-3f * r2 + 1e-12f);
+}
+
+// Descends from the root with one uniform; returns the leaf (or -1 if no
+// emitter below the root can face x) and its probability.
+__device__ int lt_sample(const SceneDev& s, int band, f3 x, float u, float& p) {
+  int node = 0;
+  p = 1.0f;
+  while (s.lnodes[node].right >= 0) {
+    const int l = s.lnodes[node].left, r = s.lnodes[node].right;
+    const float il = lt_importance(s, l, band, x), ir = lt_importance(s, r, band, x);
+    const float sum = il + ir;
+    if (!(sum > 0.0f)) return -1;
+    const float pl = il / sum;
+    if (u < pl) {
+      u = fminf(u / pl, 0x1.fffffep-1f);
+      p *= pl;
+      node = l;
+    } else {
+      const float pr = ir / sum;
+      u = fminf((u - pl) / pr, 0x1.fffffep-1f);
+      p *= pr;
+      node = r;
+    }
+  }
+  return node;
+}
+
+// Probability that lt_sample picks `leaf` from x.
+__device__ float lt_pdf(const SceneDev& s, int band, f3 x, int leaf) {
+  float p = 1.0f;
+  int c = leaf;
+  int par = s.lnodes[c].parent;
+  while (par >= 0) {
+    const int l = s.lnodes[par].left, r = s.lnodes[par].right;
+    const float il = lt_importance(s, l, band, x), ir = lt_importance(s, r, band, x);
+    const float sum = il + ir;
+    if (!(sum > 0.0f)) return 0.0f;
+    p *= (c == l ? il : ir) / sum;
+    c = par;
+    par = s.lnodes[par].parent;
+  }
+  return p;
+}
+
+__device__ __forceinline__ int lt_leaf_of(const SceneDev& s, int face) {
+  int lo = 0, hi = s.n_lkeys;
+  while (lo < hi) {
+    const int m = (lo + hi) >> 1;
+    if (s.lkeys[m].face < face) lo = m + 1;
+    else hi = m;
+  }
+  return (lo < s.n_lkeys && s.lkeys[lo].face == face) ? s.lkeys[lo].leaf : -1;
+}
+
 // Extension ray came back: scatter_volume (main.py:1190-1292) and the
 // cast_rays event dispatch (main.py:1540-1578) up to the NEE shadow walk.
 template <bool ADJ>
@@ -538,7 +632,17 @@ __device__ __forceinline__ void on_extension(Lane& L, const Ctx<ADJ>& c, const H
     if (smt.se) {  // direct hit on an emitter (main.py:1550-1559)
       float d2 = travel * travel;
       float cl = tmax(0.0f, dot(hn, -L.dir));
-      float pdfl = fdiv(d2 * tmin(1.0f, (float)L.pl), cl * s.light_area[L.active] + 1e-6f);
+      float lden = cl * s.light_area[L.active] + 1e-6f;
+      if (a.nee_tree && cl > 0.0f && L.pl > 0) {
+        // the light tree's pick probability at the previous NEE vertex; a
+        // face the band's NEE cannot pick keeps the reference's weight
+        const int leaf = lt_leaf_of(s, h.prim);
+        if (leaf >= 0 && s.lpower[(size_t)leaf * s.B + L.active] > 0.0f) {
+          const float p = lt_pdf(s, L.active, L.sdir, leaf);
+          if (p > 0.0f) lden = cl * (s.shade[(size_t)h.prim * 4].w / p) + 1e-6f;
+        }
+      }
+      float pdfl = fdiv(d2 * tmin(1.0f, (float)L.pl), lden);
       float den = pdfl * pdfl + L.sp * L.sp + 1e-6f;
       float w = fdiv(L.sp * L.sp, den);
       float contrib = L.tp * smt.em * w;
@@ -557,31 +661,50 @@ __device__ __forceinline__ void on_extension(Lane& L, const Ctx<ADJ>& c, const H
   const int nl = s.light_count[ab];
   if (nl <= 0) {
     L.fdirect = 0.0f;
+    L.sdir = L.pos;
     L.phase = PH_FIN;
     return;
   }
-  const LightRec* lr = s.lights + (size_t)ab * s.max_lights;
   const U4 rn = draws(a, L, L.seg, SR_SLOT_NEE);
-  float r = rn.u[0];
-  // first i with r < cdf[i] (main.py:1310-1313) by binary search over the
-  // non-decreasing CDF: the same light as the linear scan, O(log n) reads
-  // (0 if none, the scan's fallback)
-  int lo = 0, hi = nl;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (r < lr[mid].cdf) hi = mid;
-    else lo = mid + 1;
+  int lslot, lprim;
+  if (a.nee_tree) {
+    float p;
+    const int leaf = lt_sample(s, ab, L.pos, rn.u[0], p);
+    if (leaf < 0) {  // every light of the band faces away from this point
+      L.fdirect = 0.0f;
+      L.sdir = L.pos;
+      L.phase = PH_FIN;
+      return;
+    }
+    lslot = ~s.lnodes[leaf].left;
+    lprim = s.lnodes[leaf].face;
+    L.fdirect = s.shade[(size_t)lprim * 4].w / p;
+  } else {
+    const LightRec* lr = s.lights + (size_t)ab * s.max_lights;
+    float r = rn.u[0];
+    // first i with r < cdf[i] (main.py:1310-1313) by binary search over the
+    // non-decreasing CDF: the same light as the linear scan, O(log n) reads
+    // (0 if none, the scan's fallback)
+    int lo = 0, hi = nl;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (r < lr[mid].cdf) hi = mid;
+      else lo = mid + 1;
+    }
+    const int e = lo < nl ? lo : 0;
+    const LightRec le = lr[e];
+    lslot = le.slot;
+    lprim = le.prim;
+    L.fdirect = s.light_area[ab];
   }
-  const int e = lo < nl ? lo : 0;
-  LightRec le = lr[e];
-  const float4* lv = s.lverts + (size_t)le.slot * 3;
+  const float4* lv = s.lverts + (size_t)lslot * 3;
   f3 v0 = xyz(lv[0]), v1 = xyz(lv[1]), v2 = xyz(lv[2]);
   float bx = rn.u[1], by = rn.u[2];
   if (bx + by > 1.0f) { bx = 1.0f - bx; by = 1.0f - by; }
   f3 p = v0 + bx * (v1 - v0) + by * (v2 - v0);
   L.sdir = normalize(p - L.pos);
   L.smax = length(p - L.pos);
-  L.sprim = le.prim;
+  L.sprim = lprim;
   L.svol = L.vol;
   L.strav = 0.0f;
   L.stot = 1.0f;
@@ -632,6 +755,7 @@ __device__ __forceinline__ void on_shadow(Lane& L, const Ctx<ADJ>& c, const HitD
     }
   }
   if (!done) return;
+  const float aeff = L.fdirect;
   float direct = 0.0f;
   if (L.stot > 1e-6f) {
     const float4* shl = s.shade + (size_t)L.sprim * 4;
@@ -649,9 +773,9 @@ __device__ __forceinline__ void on_shadow(Lane& L, const Ctx<ADJ>& c, const HitD
       bsdf = hg_eval(cosv, get_mat(s, L.vol, ab).g);
       pdf = bsdf;
     }
-    float factor = bsdf * cosv * L.stot * gt * s.light_area[ab];
+    float factor = bsdf * cosv * L.stot * gt * aeff;
     float emis = get_mat(s, lmat, L.band).em;
-    float pdfl = fdiv(d2, tmax(1e-6f, cl) * s.light_area[ab]);
+    float pdfl = fdiv(d2, tmax(1e-6f, cl) * aeff);
     float w = fdiv(pdfl * pdfl, pdfl * pdfl + pdf * pdf + 1e-6f);
     direct = L.tp * emis * factor * w;
     if (ADJ && L.pass == 1 && isfinite(direct) && direct != 0.0f) {
@@ -681,6 +805,7 @@ __device__ __forceinline__ void on_shadow(Lane& L, const Ctx<ADJ>& c, const HitD
   // finish_segment runs once per loop iteration for every lane that reaches
   // it (from either handler), so its large scatter_surface body is shared
   L.fdirect = direct;
+  L.sdir = L.pos;
   L.phase = PH_FIN;
 }
 

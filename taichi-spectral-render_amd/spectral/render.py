@@ -111,7 +111,8 @@ class Renderer:
         return {"bvh_nodes": n.value, "bvh4_depth": d.value, "max_lights_per_band": m.value,
                 "device_bytes": b.value}
 
-    SCENE_ARRAYS = {"nodes": 0, "tris": 1, "shade": 2, "lights": 3, "lverts": 4, "light_count": 5, "light_area": 6}
+    SCENE_ARRAYS = {"nodes": 0, "tris": 1, "shade": 2, "lights": 3, "lverts": 4, "light_count": 5, "light_area": 6,
+                    "light_tree": 7, "light_power": 8, "light_keys": 9}
 
     def scene_array(self, name):
         """The compiled scene array `name` (SCENE_ARRAYS) as raw bytes
@@ -144,15 +145,18 @@ class Renderer:
         self.flat.cam = cam
         self.clear_pixel_space()
 
-    def render(self, n_blocks=1, max_depth=7, seed=0, stream=0):
+    def render(self, n_blocks=1, max_depth=7, seed=0, stream=0, light_tree=False):
         """Run blocks [blocks_done, blocks_done + n_blocks) and fold them into
         the running spectral mean ``accum`` [W, H, B] (cast_rays x B +
-        average_and_reset_block per block, main.py:1826-1836)."""
+        average_and_reset_block per block, main.py:1826-1836).
+        ``light_tree``: NEE picks lights through the light tree (same
+        expected image, lower variance with many lights) instead of the
+        reference's area CDF."""
         if self.accum is None:
             raise RuntimeError("set_resolution() first")
         self._bind_stream()
         _lib.call("sr_render", self._h, self.width, self.height, int(seed), int(stream), self.blocks_done,
-                  int(n_blocks), int(max_depth), 0, _lib.ptr(self.accum))
+                  int(n_blocks), int(max_depth), 1 if light_tree else 0, _lib.ptr(self.accum))
         self.blocks_done += int(n_blocks)
         return self.accum
 

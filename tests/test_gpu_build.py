@@ -20,7 +20,8 @@ from spectral.scene import flatten_scene, pixel_center_rays
 
 pytestmark = pytest.mark.gpu
 
-ARRAYS = ("nodes", "tris", "shade", "lights", "lverts", "light_count", "light_area")
+ARRAYS = ("nodes", "tris", "shade", "lights", "lverts", "light_count", "light_area", "light_tree", "light_power",
+          "light_keys")
 
 
 def _random_rays(n, seed, lo=-0.49, hi=0.49):
