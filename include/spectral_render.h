@@ -141,11 +141,16 @@ int sr_trace_closest(sr_handle* h, int n, const float* orig, const float* dir, f
  * the blocks into the running mean dev_accum [W][H][B] exactly as
  * average_and_reset_block (main.py:1592-1607) does once per block.
  * Randomness: Philox-4x32-10 keyed by seed, counter (i*H + j, block,
- * band | stream << 8, call/4).  flags: 0, or SR_RENDER_LIGHT_TREE. */
+ * band | stream << 8, call/4).  flags: 0 or SR_RENDER_LIGHT_TREE | SR_RENDER_SUBSURFACE. */
 #define SR_RENDER_LIGHT_TREE 1 /* NEE picks its light through the light tree (importance by distance and
                                   orientation; the TODO of main.py:541) instead of the area CDF
                                   (main.py:1306-1313): same expected image, lower variance with many
                                   lights.  Default 0 reproduces the reference's choice exactly. */
+#define SR_RENDER_SUBSURFACE 2 /* opaque materials with scattering_coefficient > 0 scatter by a random walk
+                                  inside the mesh (sigma_s, 4 pi k / lambda, HG g of the band; Lambertian
+                                  entry and exit) instead of the single-scatter Oren-Nayar lobe: the TODO
+                                  of main.py:1066.  Meshes must be closed (a walk that leaves an open
+                                  mesh ends).  Not in the reference; off by default. */
 int sr_render(sr_handle* h, int width, int height, uint64_t seed, uint32_t stream, int first_block, int n_blocks,
               int max_depth, int flags, float* dev_accum);
 

@@ -93,7 +93,7 @@ struct sr_handle {
   // work
   DevBuf sum, work, jobs, overflow, tmp, alb_off;
   Job job_host;  // staging copy of the one-job table (kept alive for the async copy)
-  int grid = 0, adj_grid = 0;
+  int grid = 0, adj_grid = 0, sss_grid = 0;
 };
 
 static void fill_scene_dev(const sr_handle* h, SceneDev& s) {
@@ -695,7 +695,7 @@ static int prepare_path_args(sr_handle* h, PathArgs& a, int width, int height, u
 int sr_render(sr_handle* h, int width, int height, uint64_t seed, uint32_t stream, int first_block, int n_blocks,
               int max_depth, int flags, float* dev_accum) {
   g_err.clear();
-  if (flags & ~SR_RENDER_LIGHT_TREE) return fail(SR_EINVAL, "unknown render flags 0x%x", flags);
+  if (flags & ~(SR_RENDER_LIGHT_TREE | SR_RENDER_SUBSURFACE)) return fail(SR_EINVAL, "unknown render flags 0x%x", flags);
   int rc = check_render_args(h, width, height, first_block, n_blocks, max_depth, stream);
   if (rc) return rc;
   if (!dev_accum) return fail(SR_EINVAL, "dev_accum is NULL");
@@ -706,11 +706,14 @@ int sr_render(sr_handle* h, int width, int height, uint64_t seed, uint32_t strea
                               (flags & SR_RENDER_LIGHT_TREE) ? 1 : 0)))
     return rc;
   hipStream_t st = h->st;
-  int grid = h->grid;
+  const bool sss = (flags & SR_RENDER_SUBSURFACE) != 0;
+  if (sss && h->sss_grid == 0 && sr_path_sss_grid(&h->sss_grid)) return fail(SR_EHIP, "occupancy query failed");
+  int grid = sss ? h->sss_grid : h->grid;
   const unsigned long long waves_needed = ((unsigned long long)h->job_host.n_items + 255) / 256;
   if ((unsigned long long)grid > waves_needed) grid = (int)waves_needed;
   HIPCHK(hipEventRecord(h->ev0, st));
-  if (sr_launch_path(a, grid, 256, st)) return fail(SR_EHIP, "path kernel launch failed");
+  if ((sss ? sr_launch_path_sss(a, grid, 256, st) : sr_launch_path(a, grid, 256, st)))
+    return fail(SR_EHIP, "path kernel launch failed");
   HIPCHK(hipEventRecord(h->ev1, st));
   h->timed = true;
   const size_t n = (size_t)width * height * h->B;

@@ -82,7 +82,11 @@ enum {
   SR_SLOT_NEE = 1,   // sample_direct_light: light pick, point on the light (2)
   SR_SLOT_SCAT = 2,  // scatter_surface: GGX half vector (2), Fresnel choice, subsurface distance
   SR_SLOT_OPQ = 3,   // scatter_surface, opaque: cosine direction (2), surface EEM (2)
-  SR_SLOT_AUX = 4    // camera jitter (segment 0, words 0-1); fluorescence direction (words 2-3)
+  SR_SLOT_AUX = 4,   // camera jitter (segment 0, words 0-1); fluorescence direction (words 2-3)
+  // subsurface random walk (SR_RENDER_SUBSURFACE), counter seg << 9 | step:
+  SR_SLOT_WALK = 5,      // free flight, HG direction (2)
+  SR_SLOT_WALK_NEE = 6,  // NEE at the exit point
+  SR_SLOT_WALK_EXIT = 7  // cosine exit direction (2)
 };
 struct U4 {
   float u[4];

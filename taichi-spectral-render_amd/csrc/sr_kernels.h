@@ -93,6 +93,8 @@ int sr_path_grid_bs(int block, int* grid_out);
 size_t sr_path_lds_bytes(int block, bool adjoint);
 int sr_launch_path(const PathArgs& a, int grid, int block, hipStream_t st);
 int sr_launch_adjoint(const PathArgs& a, int grid, int block, hipStream_t st);
+int sr_path_sss_grid(int* grid_out);
+int sr_launch_path_sss(const PathArgs& a, int grid, int block, hipStream_t st);
 int sr_launch_fill_u32(uint32_t* p, size_t n_words, uint32_t v, hipStream_t st);
 int sr_launch_finalize(float* mean, float* sum, size_t n, int first, int nb, hipStream_t st);
 int sr_launch_fill(float* p, size_t n, float v, hipStream_t st);

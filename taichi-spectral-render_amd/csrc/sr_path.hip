@@ -43,6 +43,9 @@ struct Lane {
   f3 sdir;
   float smax, strav, stot, soff;
   int sprim, svol, sstep;
+  // subsurface random walk (SR_RENDER_SUBSURFACE): 0 none, 1 walking inside
+  // material wmat (step wstep), 2 at the exit point (NEE, then the exit lobe)
+  int walk, wstep, wmat;
   // during the shadow walk (PH_SHD): the NEE light's area over its pick
   // probability (the band's emitting area for the CDF pick); then the NEE
   // contribution awaiting finish_segment (PH_FIN)
@@ -142,6 +145,9 @@ __device__ __forceinline__ void start_path(Lane& L, const Ctx<ADJ>& c) {
   L.seg = 0;
   L.acc = 0.0f;
   L.acc_band = L.band;
+  L.walk = 0;
+  L.wstep = 0;
+  L.wmat = 0;
   if (ADJ) {
     if (L.pass == 0) L.Lw = 0.0f;
     L.Eb = 0.0f;
@@ -157,8 +163,10 @@ __device__ __forceinline__ void start_path(Lane& L, const Ctx<ADJ>& c) {
 
 // evaluate_bsdf, main.py:1360-1444 (its transmission branch is unreachable:
 // both cosines are clamped to >= 1e-4 before NdotV * NdotL > 0 is tested).
+// With `walk` (SR_RENDER_SUBSURFACE) an opaque material with sigma_s > 0 has
+// no diffuse lobe here: that light enters the subsurface walk instead.
 __device__ __forceinline__ void evaluate_bsdf(const SceneDev& s, int vol_id, int surf_id, f3 V, f3 Lv, f3 N, int b,
-                                              float& bsdf, float& pdf) {
+                                              float& bsdf, float& pdf, bool walk = false) {
   MatRec vm = get_mat(s, vol_id, b);
   MatRec sm = get_mat(s, surf_id, b);
   float NdotL = tmax(1e-4f, dot(N, Lv));
@@ -179,7 +187,7 @@ __device__ __forceinline__ void evaluate_bsdf(const SceneDev& s, int vol_id, int
   float absorb = fdiv(4.0f * SR_PI * sm.k * 1e9f, s.lam_nm[b]);
   float albedo = fdiv(sm.ss, sm.ss + absorb + 1e-6f);
   float on = oren_nayar(sm.r, NdotL, NdotV, Lv, V, N);
-  float opq = 1.0f - (float)sm.tv;
+  float opq = (walk && sm.ss > 0.0f) ? 0.0f : 1.0f - (float)sm.tv;
   bsdf += (1.0f - F) * opq * (albedo * SR_INV_PI) * on;
   pdf += NdotL * 0.318309886184f * (1.0f - F) * opq;
 }
@@ -219,10 +227,19 @@ __device__ __forceinline__ void evaluate_bsdf_d(const SceneDev& s, const AdjMap&
 // scatter_surface, main.py:1000-1188, applied to the lane's ray as
 // cast_rays does (main.py:1565-1574).  `direct` is this segment's NEE
 // contribution (accumulated after the scatter), needed by the adjoint.
-template <bool ADJ>
+template <bool ADJ, bool SSS>
 __device__ __forceinline__ void scatter_surface(Lane& L, const Ctx<ADJ>& c, float direct) {
   const Job& a = c.a;
   const SceneDev& s = a.s;
+  if (SSS && L.walk == 2) {  // leaving a subsurface walk: Lambertian lobe about the outward normal
+    const U4 u = draws(a, L, (L.seg << 9) | L.wstep, SR_SLOT_WALK_EXIT);
+    const f3 S = cosine_hemisphere(L.en, u.u[0], u.u[1]);
+    L.sp = 0.318309886184f * tmax(1e-4f, dot(L.en, S));
+    L.dir = S;
+    L.pos = L.pos + L.en * SR_EPSILON;
+    L.walk = 0;
+    return;
+  }
   const int ab = L.active;
   MatRec sm = get_mat(s, L.emat, ab);
   MatRec vm = get_mat(s, L.vol, ab);
@@ -253,7 +270,20 @@ __device__ __forceinline__ void scatter_surface(Lane& L, const Ctx<ADJ>& c, floa
   } else {
     float absorb = fdiv(4.0f * SR_PI * kt, s.lam_m[ab]);
     float dte = fdiv(-logf(-rs.u[3] + 1.0f + 1e-7f), sm.ss + absorb + 1e-6f);
-    if (!sm.tv) {
+    if (SSS && !sm.tv && sm.ss > 0.0f) {
+      // random-walk subsurface scattering (the TODO of main.py:1066) in
+      // place of the single-scatter Oren-Nayar lobe: enter along a cosine-
+      // distributed direction into the surface; walk_step does the rest
+      const U4 ro = draws(a, L, L.seg, SR_SLOT_OPQ);
+      ndir = cosine_hemisphere(-L.en, ro.u[0], ro.u[1]);
+      est = 1.0f - F;
+      pdf = 1.0f;
+      dr = 1;  // the outside medium stays the path's medium
+      branch = 5;
+      L.walk = 1;
+      L.wstep = 0;
+      L.wmat = L.emat;
+    } else if (!sm.tv) {
       dr = 1;
       branch = 3;
       const U4 ro = draws(a, L, L.seg, SR_SLOT_OPQ);
@@ -375,12 +405,13 @@ __device__ __forceinline__ void scatter_surface(Lane& L, const Ctx<ADJ>& c, floa
   L.dir = ndir;
   L.tp *= est;
   float fdr = (float)dr;
-  L.pos = L.pos + L.en * SR_EPSILON * (fdr - (1.0f - fdr));
+  const float side = (SSS && branch == 5) ? -1.0f : (fdr - (1.0f - fdr));  // a walk starts inside
+  L.pos = L.pos + L.en * SR_EPSILON * side;
   L.vol = L.emat * L.efront * (1 - dr) + L.vol * dr;
 }
 
 // End of one path-loop iteration (main.py:1565-1590).
-template <bool ADJ>
+template <bool ADJ, bool SSS>
 __device__ __forceinline__ void finish_segment(Lane& L, const Ctx<ADJ>& c, float direct) {
   if (ADJ && L.pass == 1 && L.hg_sc != 0.0f) {
     float R = L.Lw - L.Eb - (isfinite(direct) ? dacc(c, L, L.active) * direct : 0.0f);
@@ -389,7 +420,7 @@ __device__ __forceinline__ void finish_segment(Lane& L, const Ctx<ADJ>& c, float
   }
   if (L.eaction == 1) {
     L.pl += 1;
-    scatter_surface<ADJ>(L, c, direct);
+    scatter_surface<ADJ, SSS>(L, c, direct);
   } else if (L.eaction == 2) {
     L.dir = L.edir;
     L.pl += 1;
@@ -501,10 +532,125 @@ __device__ __forceinline__ int lt_leaf_of(const SceneDev& s, int face) {
   return (lo < s.n_lkeys && s.lkeys[lo].face == face) ? s.lkeys[lo].leaf : -1;
 }
 
+// sample_direct_light, main.py:1296-1326, up to the shadow walk: pick a
+// light and a point on it with the uniform block (seg, slot).
+template <bool ADJ>
+__device__ __forceinline__ void nee_setup(Lane& L, const Ctx<ADJ>& c, int surf, int seg, int slot) {
+  const Job& a = c.a;
+  const SceneDev& s = a.s;
+  const int ab = L.active;
+  L.nee_surf = surf;
+  const int nl = s.light_count[ab];
+  if (nl <= 0) {
+    L.fdirect = 0.0f;
+    L.sdir = L.pos;
+    L.phase = PH_FIN;
+    return;
+  }
+  const U4 rn = draws(a, L, seg, slot);
+  int lslot, lprim;
+  if (a.nee_tree) {
+    float p;
+    const int leaf = lt_sample(s, ab, L.pos, rn.u[0], p);
+    if (leaf < 0) {  // every light of the band faces away from this point
+      L.fdirect = 0.0f;
+      L.sdir = L.pos;
+      L.phase = PH_FIN;
+      return;
+    }
+    lslot = ~s.lnodes[leaf].left;
+    lprim = s.lnodes[leaf].face;
+    L.fdirect = s.shade[(size_t)lprim * 4].w / p;
+  } else {
+    const LightRec* lr = s.lights + (size_t)ab * s.max_lights;
+    float r = rn.u[0];
+    // first i with r < cdf[i] (main.py:1310-1313) by binary search over the
+    // non-decreasing CDF: the same light as the linear scan, O(log n) reads
+    // (0 if none, the scan's fallback)
+    int lo = 0, hi = nl;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (r < lr[mid].cdf) hi = mid;
+      else lo = mid + 1;
+    }
+    const int e = lo < nl ? lo : 0;
+    const LightRec le = lr[e];
+    lslot = le.slot;
+    lprim = le.prim;
+    L.fdirect = s.light_area[ab];
+  }
+  const float4* lv = s.lverts + (size_t)lslot * 3;
+  f3 v0 = xyz(lv[0]), v1 = xyz(lv[1]), v2 = xyz(lv[2]);
+  float bx = rn.u[1], by = rn.u[2];
+  if (bx + by > 1.0f) { bx = 1.0f - bx; by = 1.0f - by; }
+  f3 p = v0 + bx * (v1 - v0) + by * (v2 - v0);
+  L.sdir = normalize(p - L.pos);
+  L.smax = length(p - L.pos);
+  L.sprim = lprim;
+  L.svol = L.vol;
+  L.strav = 0.0f;
+  L.stot = 1.0f;
+  L.soff = 0.0f;
+  L.sstep = 0;
+  if (ADJ) L.stot_dsig = 0.0f;
+  L.phase = PH_SHD;
+}
+
+// One step of a subsurface random walk inside material wmat (sigma_s, the
+// absorption 4 pi k / lambda and the HG g of its band, as the reference's
+// opaque branch forms them, main.py:1057-1058): a free flight; a boundary
+// hit before it ends is the exit point, which acts as a Lambertian vertex
+// (NEE, then a cosine lobe about the outward normal); otherwise an HG
+// scatter with weight sigma_s / sigma_t.  A walk that misses every surface
+// has left an open mesh and ends; kMaxWalk steps end it too.
+constexpr int kMaxWalk = 256;
+
+template <bool ADJ>
+__device__ __forceinline__ void walk_step(Lane& L, const Ctx<ADJ>& c, const HitD& h) {
+  const Job& a = c.a;
+  const SceneDev& s = a.s;
+  L.n_trav++;
+  const int ab = L.active;
+  const MatRec wm = get_mat(s, L.wmat, ab);
+  const float sa = fdiv(4.0f * SR_PI * wm.k, s.lam_m[ab]);
+  const float stt = wm.ss + sa;
+  const int code = (L.seg << 9) | L.wstep;
+  const U4 u = draws(a, L, code, SR_SLOT_WALK);
+  const float t = -logf(1.0f - u.u[0]) / stt;
+  if (h.prim < 0) {
+    end_path<ADJ>(L, c);
+    return;
+  }
+  if (h.t <= t) {
+    L.pos = L.pos + L.dir * h.t;
+    const SurfHit sh = surface_of(s, h, L.dir);
+    const f3 no = dot(sh.normal, L.dir) > 0.0f ? sh.normal : -sh.normal;
+    L.en = no;
+    L.esn = no;
+    L.efront = 1;
+    L.eaction = 1;
+    L.walk = 2;
+    nee_setup<ADJ>(L, c, -2, code, SR_SLOT_WALK_NEE);
+    return;
+  }
+  L.pos = L.pos + L.dir * t;
+  L.tp *= wm.ss / stt;
+  L.dir = hg_sample(L.dir, wm.g, u.u[1], u.u[2]);
+  if (++L.wstep >= kMaxWalk) {
+    end_path<ADJ>(L, c);
+    return;
+  }
+  L.phase = PH_EXT;
+}
+
 // Extension ray came back: scatter_volume (main.py:1190-1292) and the
 // cast_rays event dispatch (main.py:1540-1578) up to the NEE shadow walk.
-template <bool ADJ>
+template <bool ADJ, bool SSS>
 __device__ __forceinline__ void on_extension(Lane& L, const Ctx<ADJ>& c, const HitD& h) {
+  if (SSS && L.walk == 1) {
+    walk_step<ADJ>(L, c, h);
+    return;
+  }
   const Job& a = c.a;
   const SceneDev& s = a.s;
   L.n_seg++;
@@ -656,67 +802,12 @@ __device__ __forceinline__ void on_extension(Lane& L, const Ctx<ADJ>& c, const H
     }
     surf = L.emat;
   }
-  // sample_direct_light, main.py:1296-1326 (up to the shadow walk)
-  L.nee_surf = surf;
-  const int nl = s.light_count[ab];
-  if (nl <= 0) {
-    L.fdirect = 0.0f;
-    L.sdir = L.pos;
-    L.phase = PH_FIN;
-    return;
-  }
-  const U4 rn = draws(a, L, L.seg, SR_SLOT_NEE);
-  int lslot, lprim;
-  if (a.nee_tree) {
-    float p;
-    const int leaf = lt_sample(s, ab, L.pos, rn.u[0], p);
-    if (leaf < 0) {  // every light of the band faces away from this point
-      L.fdirect = 0.0f;
-      L.sdir = L.pos;
-      L.phase = PH_FIN;
-      return;
-    }
-    lslot = ~s.lnodes[leaf].left;
-    lprim = s.lnodes[leaf].face;
-    L.fdirect = s.shade[(size_t)lprim * 4].w / p;
-  } else {
-    const LightRec* lr = s.lights + (size_t)ab * s.max_lights;
-    float r = rn.u[0];
-    // first i with r < cdf[i] (main.py:1310-1313) by binary search over the
-    // non-decreasing CDF: the same light as the linear scan, O(log n) reads
-    // (0 if none, the scan's fallback)
-    int lo = 0, hi = nl;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (r < lr[mid].cdf) hi = mid;
-      else lo = mid + 1;
-    }
-    const int e = lo < nl ? lo : 0;
-    const LightRec le = lr[e];
-    lslot = le.slot;
-    lprim = le.prim;
-    L.fdirect = s.light_area[ab];
-  }
-  const float4* lv = s.lverts + (size_t)lslot * 3;
-  f3 v0 = xyz(lv[0]), v1 = xyz(lv[1]), v2 = xyz(lv[2]);
-  float bx = rn.u[1], by = rn.u[2];
-  if (bx + by > 1.0f) { bx = 1.0f - bx; by = 1.0f - by; }
-  f3 p = v0 + bx * (v1 - v0) + by * (v2 - v0);
-  L.sdir = normalize(p - L.pos);
-  L.smax = length(p - L.pos);
-  L.sprim = lprim;
-  L.svol = L.vol;
-  L.strav = 0.0f;
-  L.stot = 1.0f;
-  L.soff = 0.0f;
-  L.sstep = 0;
-  if (ADJ) L.stot_dsig = 0.0f;
-  L.phase = PH_SHD;
+  nee_setup<ADJ>(L, c, surf, L.seg, SR_SLOT_NEE);
 }
 
 // Shadow ray came back: one step of shadow_transmittance (main.py:1456-1498);
 // on the last step, the rest of sample_direct_light (main.py:1329-1356).
-template <bool ADJ>
+template <bool ADJ, bool SSS>
 __device__ __forceinline__ void on_shadow(Lane& L, const Ctx<ADJ>& c, const HitD& h) {
   const Job& a = c.a;
   const SceneDev& s = a.s;
@@ -765,9 +856,13 @@ __device__ __forceinline__ void on_shadow(Lane& L, const Ctx<ADJ>& c, const HitD
     float d2 = L.smax * L.smax;
     float gt = fdiv(cl, d2 + 1e-4f);
     float bsdf, cosv, pdf;
-    if (L.nee_surf != -1) {
+    if (SSS && L.nee_surf == -2) {  // exit point of a subsurface walk: Lambertian
       cosv = tmax(0.0f, dot(L.esn, L.sdir));
-      evaluate_bsdf(s, L.vol, L.nee_surf, -L.dir, L.sdir, L.esn, ab, bsdf, pdf);
+      bsdf = SR_INV_PI;
+      pdf = cosv * SR_INV_PI;
+    } else if (L.nee_surf != -1) {
+      cosv = tmax(0.0f, dot(L.esn, L.sdir));
+      evaluate_bsdf(s, L.vol, L.nee_surf, -L.dir, L.sdir, L.esn, ab, bsdf, pdf, SSS);
     } else {
       cosv = tmax(0.0f, dot(L.dir, L.sdir));
       bsdf = hg_eval(cosv, get_mat(s, L.vol, ab).g);
@@ -840,7 +935,7 @@ __device__ __forceinline__ void flush_counters(const PathArgs& P, int j, Lane& L
   L.n_seg = L.n_trav = L.n_nf = 0;
 }
 
-template <bool ADJ>
+template <bool ADJ, bool SSS>
 __device__ __forceinline__ void path_loop(const PathArgs& P, int* stk, float* lds_grad) {
   const int stride = blockDim.x;
   const int lane = (int)__lane_id();
@@ -957,10 +1052,10 @@ __device__ __forceinline__ void path_loop(const PathArgs& P, int* stk, float* ld
       const Node4* nodes = J.s.nodes;
       const float4* tris = J.s.tris;
       HitD h = trace_closest(nodes, tris, ro, rd, stk, stride, P.overflow, t_init);
-      if (do_ext) on_extension<ADJ>(L, c, h);
-      else on_shadow<ADJ>(L, c, h);
+      if (do_ext) on_extension<ADJ, SSS>(L, c, h);
+      else on_shadow<ADJ, SSS>(L, c, h);
     }
-    if (!do_ext && L.phase == PH_FIN) finish_segment<ADJ>(L, c, L.fdirect);
+    if (!do_ext && L.phase == PH_FIN) finish_segment<ADJ, SSS>(L, c, L.fdirect);
   }
 }
 
@@ -971,7 +1066,14 @@ __device__ __forceinline__ void path_loop(const PathArgs& P, int* stk, float* ld
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void sr_path_kernel(PathArgs a) {
   extern __shared__ int lds_dyn[];
   if (threadIdx.x == 0) atomicMin(a.clocks, wall_clock64());
-  path_loop<false>(a, lds_dyn + threadIdx.x, nullptr);
+  path_loop<false, false>(a, lds_dyn + threadIdx.x, nullptr);
+}
+
+// The same path loop with random-walk subsurface scattering (SR_RENDER_SUBSURFACE).
+__global__ __launch_bounds__(256) void sr_path_sss_kernel(PathArgs a) {
+  extern __shared__ int lds_dyn[];
+  if (threadIdx.x == 0) atomicMin(a.clocks, wall_clock64());
+  path_loop<false, true>(a, lds_dyn + threadIdx.x, nullptr);
 }
 
 __global__ __launch_bounds__(256) void sr_adjoint_kernel(PathArgs a) {
@@ -979,7 +1081,7 @@ __global__ __launch_bounds__(256) void sr_adjoint_kernel(PathArgs a) {
   float* lds_grad = reinterpret_cast<float*>(lds_dyn + SR_STACK_SIZE * blockDim.x);
   for (int i = threadIdx.x; i < a.adj.n_params; i += blockDim.x) lds_grad[i] = 0.0f;
   __syncthreads();
-  path_loop<true>(a, lds_dyn + threadIdx.x, lds_grad);
+  path_loop<true, false>(a, lds_dyn + threadIdx.x, lds_grad);
   __syncthreads();
   for (int i = threadIdx.x; i < a.adj.n_params; i += blockDim.x) {
     float v = lds_grad[i];
@@ -1019,6 +1121,15 @@ int sr_adjoint_grid(int* grid_out) {
 
 int sr_launch_path(const PathArgs& a, int grid, int block, hipStream_t st) {
   hipLaunchKernelGGL(sr_path_kernel, dim3(grid), dim3(block), sr_path_lds_bytes(block, false), st, a);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int sr_path_sss_grid(int* grid_out) {
+  return grid_for((const void*)sr_path_sss_kernel, 256, sr_path_lds_bytes(256, false), grid_out);
+}
+
+int sr_launch_path_sss(const PathArgs& a, int grid, int block, hipStream_t st) {
+  hipLaunchKernelGGL(sr_path_sss_kernel, dim3(grid), dim3(block), sr_path_lds_bytes(block, false), st, a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

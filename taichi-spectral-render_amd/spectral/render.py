@@ -145,18 +145,21 @@ class Renderer:
         self.flat.cam = cam
         self.clear_pixel_space()
 
-    def render(self, n_blocks=1, max_depth=7, seed=0, stream=0, light_tree=False):
+    def render(self, n_blocks=1, max_depth=7, seed=0, stream=0, light_tree=False, subsurface=False):
         """Run blocks [blocks_done, blocks_done + n_blocks) and fold them into
         the running spectral mean ``accum`` [W, H, B] (cast_rays x B +
         average_and_reset_block per block, main.py:1826-1836).
         ``light_tree``: NEE picks lights through the light tree (same
         expected image, lower variance with many lights) instead of the
-        reference's area CDF."""
+        reference's area CDF.  ``subsurface``: opaque materials with a
+        scattering coefficient scatter by a random walk inside their (closed)
+        mesh instead of the single-scatter Oren-Nayar lobe."""
         if self.accum is None:
             raise RuntimeError("set_resolution() first")
         self._bind_stream()
         _lib.call("sr_render", self._h, self.width, self.height, int(seed), int(stream), self.blocks_done,
-                  int(n_blocks), int(max_depth), 1 if light_tree else 0, _lib.ptr(self.accum))
+                  int(n_blocks), int(max_depth), (1 if light_tree else 0) | (2 if subsurface else 0),
+                  _lib.ptr(self.accum))
         self.blocks_done += int(n_blocks)
         return self.accum
 

@@ -129,4 +129,4 @@ def test_unknown_render_flag_rejected(gpu):
     cfg, flat = single_light_scene()
     r = Renderer(flat, cfg.bands, cfg.wl_min, cfg.wl_max, width=8, height=8, device=gpu)
     with pytest.raises(_lib.SpectralRenderError, match="flags"):
-        _lib.call("sr_render", r._h, 8, 8, 0, 0, 0, 1, 4, 2, _lib.ptr(r.accum))
+        _lib.call("sr_render", r._h, 8, 8, 0, 0, 0, 1, 4, 4, _lib.ptr(r.accum))
