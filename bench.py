@@ -268,6 +268,68 @@ def bench_build(reps=3, log=print):
     return out
 
 
+def bench_rows(reps=5, log=print):
+    """Throughput of the rows around the path: the display path on
+    heightfield_1m at 1024x1024 (normals view: one closed-hit trace per
+    pixel; RGBA8 conversion: 12 B read + 4 B written per pixel), NEE through
+    the light tree against the area CDF on 128 lamps (256x256, depth 4), and
+    the subsurface walk on the closed slab (256x256, w = 0.9)."""
+    import numpy as np
+    import torch
+
+    from spectral import configs
+    from spectral.render import Renderer
+    from spectral.scene import flatten_scene
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        best = 1e9
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            best = min(best, e0.elapsed_time(e1))
+        return best
+
+    out = {"metric": "display path, light-tree NEE and subsurface walk throughput"}
+    cfg = configs.heightfield_1m()
+    flat = flatten_scene(cfg.scene, cfg.bands, cfg.wl_min, cfg.wl_max)
+    r = Renderer(flat, cfg.bands, cfg.wl_min, cfg.wl_max, width=1024, height=1024, device="cuda:0")
+    box = {}
+    ms_n = timed(lambda: box.__setitem__("rgb", r.visualize_normals(1024, 1024)))
+    ms_c = timed(lambda: r.to_rgba8(box["rgb"]))
+    npx = 1024 * 1024
+    out["normals_view"] = {"pixels": npx, "ms": round(ms_n, 3), "Mpixels_per_s": round(npx / ms_n / 1e3, 1)}
+    gbs = npx * 16 / ms_c / 1e6
+    out["rgba8"] = {"pixels": npx, "ms": round(ms_c, 4), "GB_per_s": round(gbs, 1), "frac_of_8TBps": round(gbs / 8000, 3)}
+    del r
+    for name, cfgf, kw in (("light_tree", configs.lamps, {}), ("subsurface", configs.subsurface_slab, {})):
+        c = cfgf(width=256, height=256)
+        f = flatten_scene(c.scene, c.bands, c.wl_min, c.wl_max)
+        rr = Renderer(f, c.bands, c.wl_min, c.wl_max, width=256, height=256, device="cuda:0")
+        modes = (("cdf", {}), ("tree", {"light_tree": True})) if name == "light_tree" else \
+            (("walk", {"subsurface": True}),)
+        res = {"workload": f"{c.name}: {f.n_tris} tris, {c.bands} bands, 256x256, depth {c.depth}"}
+        for mode, flags in modes:
+            rr.render(n_blocks=1, max_depth=c.depth, seed=c.seed, **flags)
+            rr.sync()
+            ms = []
+            for k in range(reps):
+                rr.clear_pixel_space()
+                rr.render(n_blocks=1, max_depth=c.depth, seed=c.seed + 1 + k, **flags)
+                ms.append(rr.last_kernel_ms())
+            st = rr.stats()
+            m = float(np.median(ms))
+            res[mode] = {"kernel_ms_per_block": round(m, 3), "Mseg_per_s": round(st["segments"] / m / 1e3, 1),
+                         "traversals_per_segment": round(st["traversals"] / max(st["segments"], 1), 3)}
+        out[name] = res
+        del rr
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -297,6 +359,9 @@ def main():
         return
     if args.workload == "trace":
         print(json.dumps(bench_trace()), flush=True)
+        return
+    if args.workload == "rows":
+        print(json.dumps(bench_rows()), flush=True)
         return
     if args.workload == "build":
         print(json.dumps(bench_build()), flush=True)

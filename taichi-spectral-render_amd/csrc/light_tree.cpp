@@ -23,6 +23,8 @@
 
 namespace {
 
+constexpr double kMargin = 2e-3;  // radians
+
 struct Cone {
   double ax[3];
   double th;  // half angle, pi = every direction
@@ -112,8 +114,11 @@ struct Builder {
     }
     const Cone& c = ncone[me];
     for (int k = 0; k < 3; ++k) nd.axis[k] = (float)c.ax[k];
-    // rounded up, plus the float error of the stored axis
-    nd.theta_o = std::nextafter((float)std::min(c.th + 1e-6, 3.14159265358979323846), 4.0f);
+    // the half angle widened by kMargin (f32 rounding of the stored axis,
+    // of the cosines and of the kernel's angle arithmetic), as cos and sin
+    const double th = std::min(c.th + kMargin, 3.14159265358979323846);
+    nd.cos_o = (float)std::cos(th);
+    nd.sin_o = (float)std::sin(th);
     nodes[me] = nd;
     return me;
   }

@@ -47,19 +47,21 @@ struct alignas(16) LightRec {
 };
 
 // Light tree node (many-light NEE, SR_RENDER_LIGHT_TREE): bounding box and
-// normal cone (axis, half angle theta_o, rounded up) of the emitters below;
+// normal cone of the emitters below -- axis and the cosine and sine of its
+// half angle widened by a safety margin (cos_o = -1: every direction);
 // inner: left/right children; leaf: left = ~slot, right = -1, face = its
 // original face id.  Per-band emitting area in a separate [nodes][B] array.
 struct alignas(16) LightNode {
   float lo[3];
-  float theta_o;
+  float cos_o;
   float hi[3];
   int left;
   float axis[3];
   int right;
   int parent;  // -1 at the root
   int face;    // leaf: original face id; inner: -1
-  int pad[2];
+  float sin_o;
+  int pad;
 };
 static_assert(sizeof(LightNode) == 64, "LightNode is 64 bytes");
 

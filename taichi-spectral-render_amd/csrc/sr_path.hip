@@ -448,15 +448,15 @@ __device__ __forceinline__ void finish_segment(Lane& L, const Ctx<ADJ>& c, float
 // point x is its emitting area in the band times a conservative bound of
 // the emitters' cosine towards x over the squared distance: the angle
 // between the node's normal cone and the direction to x, reduced by the
-// cone half angle, the angle the bounding sphere subtends and a margin for
-// f32 rounding, is clamped at 0; an importance of 0 means every emitter
-// below faces away from x (cos_theta_light <= 0, main.py:1333).  The pick
+// cone half angle (widened by a margin for f32 rounding when the tree is
+// built) and by the angle the bounding sphere subtends, is clamped at 0;
+// its cosine comes from angle-difference identities (no inverse
+// trigonometry).  An importance of 0 means every emitter below faces away
+// from x (cos_theta_light <= 0, main.py:1333).  The pick
 // probability p of a light then replaces its area fraction: the estimator
 // and the MIS pdfs use A_light / p where the reference uses the band's
 // total emitting area (main.py:1351, 1355, 1556), which is the same thing
 // for the area-CDF pick.
-constexpr float kLtMargin = 2e-3f;
-
 __device__ __forceinline__ float lt_importance(const SceneDev& s, int node, int band, f3 x) {
   const float P = s.lpower[(size_t)node * s.B + band];
   if (!(P > 0.0f)) return 0.0f;
@@ -469,12 +469,17 @@ __device__ __forceinline__ float lt_importance(const SceneDev& s, int node, int 
   float cb = 1.0f;  // inside the bounding sphere: no angular bound
   if (d2 > r2) {
     const float d = sqrtf(d2);
-    const float ct = dot(mk(nd.axis[0], nd.axis[1], nd.axis[2]), dv) / d;
-    const float th = acosf(fminf(1.0f, fmaxf(-1.0f, ct)));
-    const float thu = asinf(fminf(1.0f, sqrtf(r2 / d2)));
-    const float thp = fmaxf(0.0f, th - nd.theta_o - thu - kLtMargin);
-    if (thp >= 0.5f * SR_PI) return 0.0f;
-    cb = cosf(thp);
+    const float ct = fminf(1.0f, fmaxf(-1.0f, dot(mk(nd.axis[0], nd.axis[1], nd.axis[2]), dv) / d));
+    if (ct < nd.cos_o) {  // x outside the normal cone: theta - theta_o in (0, pi]
+      const float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+      const float cd = ct * nd.cos_o + st * nd.sin_o;  // cos(theta - theta_o)
+      const float sd = st * nd.cos_o - ct * nd.sin_o;  // sin(theta - theta_o)
+      const float su = sqrtf(r2 / d2), cu = sqrtf(fmaxf(0.0f, 1.0f - r2 / d2));
+      if (cd < cu) {  // beyond the sphere's angle too
+        cb = cd * cu + sd * su;  // cos(theta - theta_o - theta_u)
+        if (cb <= 0.0f) return 0.0f;
+      }
+    }
   }
   // distance to the centre, floored for points near the centre of a large node
   return P * cb / fmaxf(d2, 1e-3f * r2 + 1e-12f);

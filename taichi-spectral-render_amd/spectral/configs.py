@@ -423,6 +423,68 @@ def batch64(n_scenes=64, seed=2, bands=31, wl_min=400.0, wl_max=700.0, tri_lo=1_
     return out
 
 
+def closed_box(lo, hi, material_id, outward=True):
+    """A closed axis-aligned box mesh, every triangle's normal pointing
+    outward (or inward, for an enclosing emitter)."""
+    x0, y0, z0 = lo
+    x1, y1, z1 = hi
+    v = [[x0, y0, z0], [x1, y0, z0], [x1, y1, z0], [x0, y1, z0], [x0, y0, z1], [x1, y0, z1], [x1, y1, z1],
+         [x0, y1, z1]]
+    quads = [(0, 1, 2, 3), (4, 5, 6, 7), (0, 1, 5, 4), (3, 2, 6, 7), (0, 3, 7, 4), (1, 2, 6, 5)]
+    c = np.mean(v, axis=0)
+    faces = []
+    for q in quads:
+        for tri in ((q[0], q[1], q[2]), (q[0], q[2], q[3])):
+            p = np.array([v[i] for i in tri], np.float64)
+            n = np.cross(p[1] - p[0], p[2] - p[0])
+            out = np.dot(n, p.mean(0) - c) > 0
+            faces.append(list(tri) if out == outward else [tri[0], tri[2], tri[1]])
+    return {"vertices": v, "faces": faces, "material_id": material_id}
+
+
+def lamps(bands=25, wl_min=400.0, wl_max=700.0, width=32, height=32, spp=16, depth=4, seed=1, n=8, h=0.3,
+          size=0.1):
+    """Many-light scene: n x n small downward-facing lamps (emission 10) at
+    height h over a 4 x 4 diffuse floor whose index matches the air
+    (Fresnel 0), camera looking down at it."""
+    wl = band_wavelengths(wl_min, wl_max, bands)
+    mats = [_mat(bands, n=1.0, tv=1),
+            _mat(bands, n=1.0, k=albedo_to_k(np.full(bands, 0.7), wl), r=WALL_ROUGHNESS, tv=0, ss=1.0),
+            _mat(bands, n=1.0, se=1, em=10.0)]
+    meshes = [{"vertices": [[-2, 0, -2], [-2, 0, 2], [2, 0, 2], [2, 0, -2]], "faces": [[0, 1, 2], [0, 2, 3]],
+               "material_id": 1}]
+    for i in range(n):
+        for j in range(n):
+            x0 = -1.75 + 3.5 * (i + 0.5) / n - size / 2
+            z0 = -1.75 + 3.5 * (j + 0.5) / n - size / 2
+            meshes.append({"vertices": [[x0, h, z0], [x0 + size, h, z0], [x0 + size, h, z0 + size],
+                                        [x0, h, z0 + size]], "faces": [[0, 1, 2], [0, 2, 3]], "material_id": 2})
+    scene = {"materials": mats, "meshes": meshes,
+             "camera": {"origin": [0.0, 2.5, 2.5], "lookat": [0.0, 0.0, 0.0], "up": [0.0, 1.0, 0.0]}}
+    return RenderConfig("lamps", scene, bands, wl_min, wl_max, width, height, spp, depth, seed)
+
+
+def subsurface_slab(w=0.9, sigma_s=5.0, bands=8, wl_min=400.0, wl_max=700.0, width=16, height=16, spp=256, depth=5,
+                    seed=4, body=None):
+    """A closed 20 x 20 x 5 slab (or `body`) of an opaque, index-matched,
+    isotropically scattering material with single albedo w (sigma_a =
+    sigma_s (1 - w) / w entered as k = sigma_a lambda / (4 pi)) inside a
+    100-unit box that emits radiance 1 inward: the semi-infinite-medium
+    setting of the subsurface walk."""
+    wl = band_wavelengths(wl_min, wl_max, bands)
+    sigma_a = sigma_s * (1.0 - w) / w
+    k = sigma_a * wl * 1e-9 / (4.0 * math.pi)
+    mats = [_mat(bands, n=1.0, tv=1),
+            _mat(bands, n=1.0, k=k, r=0.0, tv=0, ss=sigma_s, g=0.0),
+            _mat(bands, n=1.0, se=1, em=1.0)]
+    if body is None:
+        body = closed_box([-10, -5, -10], [10, 0, 10], 1)
+    meshes = [body, closed_box([-50, -50, -50], [50, 50, 50], 2, outward=False)]
+    cam = {"origin": [0.0, 3.0, 3.0], "lookat": [0.0, 0.0, 0.0], "up": [0.0, 1.0, 0.0]}
+    scene = {"materials": mats, "meshes": meshes, "camera": cam}
+    return RenderConfig("subsurface_slab", scene, bands, wl_min, wl_max, width, height, spp, depth, seed)
+
+
 WORKLOADS = {
     "box_icosphere": box_icosphere,
     "heightfield_1m": heightfield_1m,

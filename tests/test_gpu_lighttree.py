@@ -18,35 +18,15 @@ import pytest
 
 from oracle.oracle import OracleScene
 from spectral import _lib, configs
-from spectral.data import band_wavelengths
 from spectral.render import Renderer
 from spectral.scene import flatten_scene
 
 pytestmark = pytest.mark.gpu
 
 
-def lamps_scene(B=25, n=8, h=0.3, size=0.1):
-    """n x n small downward-facing lamps at height h over a 4 x 4 diffuse
-    floor whose index matches the air (Fresnel 0, so only the cosine-sampled
-    Oren-Nayar branch runs), camera looking down at it.  Every lamp is at
-    least h from every receiver."""
-    wl = band_wavelengths(400.0, 700.0, B)
-    mats = [configs._mat(B, n=1.0, tv=1),
-            configs._mat(B, n=1.0, k=configs.albedo_to_k(np.full(B, 0.7), wl), r=configs.WALL_ROUGHNESS, tv=0,
-                         ss=1.0),
-            configs._mat(B, n=1.0, se=1, em=10.0)]
-    meshes = [{"vertices": [[-2, 0, -2], [-2, 0, 2], [2, 0, 2], [2, 0, -2]], "faces": [[0, 1, 2], [0, 2, 3]],
-               "material_id": 1}]
-    for i in range(n):
-        for j in range(n):
-            x0 = -1.75 + 3.5 * (i + 0.5) / n - size / 2
-            z0 = -1.75 + 3.5 * (j + 0.5) / n - size / 2
-            meshes.append({"vertices": [[x0, h, z0], [x0 + size, h, z0], [x0 + size, h, z0 + size],
-                                        [x0, h, z0 + size]], "faces": [[0, 1, 2], [0, 2, 3]], "material_id": 2})
-    scene = {"materials": mats, "meshes": meshes,
-             "camera": {"origin": [0.0, 2.5, 2.5], "lookat": [0.0, 0.0, 0.0], "up": [0.0, 1.0, 0.0]}}
-    return configs.RenderConfig("lamps", scene, B, 400.0, 700.0, 32, 32, 16, 4, 1), flatten_scene(scene, B, 400.0,
-                                                                                                  700.0)
+def lamps_scene():
+    cfg = configs.lamps()
+    return cfg, flatten_scene(cfg.scene, cfg.bands, cfg.wl_min, cfg.wl_max)
 
 
 def single_light_scene():

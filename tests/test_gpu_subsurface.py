@@ -20,12 +20,10 @@ import numpy as np
 import pytest
 
 from spectral import configs
-from spectral.data import band_wavelengths
 from spectral.render import Renderer
 from spectral.scene import flatten_scene
 
 B = 8
-WL = band_wavelengths(400.0, 700.0, B)
 
 
 def chandrasekhar_h(w, n=96, iters=20000):
@@ -65,37 +63,13 @@ def test_h_function_solver():
         assert float((wt * h).sum()) == pytest.approx(2.0 / w * (1.0 - math.sqrt(1.0 - w)), rel=1e-9)
 
 
-def _box(lo, hi, mat, outward=True):
-    """Closed axis-aligned box, triangles oriented with normals outward (or inward)."""
-    x0, y0, z0 = lo
-    x1, y1, z1 = hi
-    v = [[x0, y0, z0], [x1, y0, z0], [x1, y1, z0], [x0, y1, z0], [x0, y0, z1], [x1, y0, z1], [x1, y1, z1],
-         [x0, y1, z1]]
-    quads = [(0, 1, 2, 3), (4, 5, 6, 7), (0, 1, 5, 4), (3, 2, 6, 7), (0, 3, 7, 4), (1, 2, 6, 5)]
-    c = np.mean(v, axis=0)
-    faces = []
-    for q in quads:
-        for tri in ((q[0], q[1], q[2]), (q[0], q[2], q[3])):
-            p = np.array([v[i] for i in tri], np.float64)
-            n = np.cross(p[1] - p[0], p[2] - p[0])
-            out = np.dot(n, p.mean(0) - c) > 0
-            faces.append(list(tri) if out == outward else [tri[0], tri[2], tri[1]])
-    return {"vertices": v, "faces": faces, "material_id": mat}
-
-
 def _scene(body, w, sigma_s):
-    sigma_a = sigma_s * (1.0 - w) / w
-    k = sigma_a * WL * 1e-9 / (4.0 * math.pi)
-    mats = [configs._mat(B, n=1.0, tv=1),
-            # smooth, index-matched boundary (roughness 0: the reference's
-            # GGX half-vector sampling at a rough boundary can pick V.H <= 0,
-            # where its Fresnel term returns ~1 and the path ends)
-            configs._mat(B, n=1.0, k=k, r=0.0, tv=0, ss=sigma_s, g=0.0),
-            configs._mat(B, n=1.0, se=1, em=1.0)]
-    meshes = [body, _box([-50, -50, -50], [50, 50, 50], 2, outward=False)]
-    cam = {"origin": [0.0, 3.0, 3.0], "lookat": [0.0, 0.0, 0.0], "up": [0.0, 1.0, 0.0]}
-    scene = {"materials": mats, "meshes": meshes, "camera": cam}
-    return flatten_scene(scene, B, 400.0, 700.0)
+    cfg = configs.subsurface_slab(w=w, sigma_s=sigma_s, bands=B, body=body)
+    return flatten_scene(cfg.scene, cfg.bands, cfg.wl_min, cfg.wl_max)
+
+
+def _box(lo, hi, mat, outward=True):
+    return configs.closed_box(lo, hi, mat, outward)
 
 
 def _mean_and_se(r, n_chunks=16, per=16, depth=5):
