@@ -213,9 +213,11 @@ typedef struct sr_batch sr_batch;
 
 int sr_batch_create(sr_batch** out);
 int sr_batch_destroy(sr_batch* b);
-/* Adds a job with sr_render's arguments.  A handle may appear once. */
+/* Adds a job with sr_render's arguments (flags: SR_RENDER_*; a batch with a
+ * SR_RENDER_SUBSURFACE job runs the walk kernel, which walks only in the
+ * jobs that ask for it).  A handle may appear once. */
 int sr_batch_submit(sr_batch* b, sr_handle* h, int width, int height, uint64_t seed, uint32_t stream,
-                    int first_block, int n_blocks, int max_depth, float* dev_accum);
+                    int first_block, int n_blocks, int max_depth, int flags, float* dev_accum);
 int sr_batch_clear(sr_batch* b);
 /* Launch shape: workgroup size (64/128/256), grid as a percentage of the
  * occupancy-limited resident grid, items claimed per queue atomic. */

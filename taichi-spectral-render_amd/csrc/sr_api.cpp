@@ -182,6 +182,7 @@ struct sr_batch {
     uint64_t seed;
     uint32_t stream;
     float* accum;
+    int flags;  // SR_RENDER_*
   };
   std::vector<Spec> specs;
   std::vector<Job> host_jobs;
@@ -668,12 +669,13 @@ static int ensure_sum(sr_handle* h, int width, int height, hipStream_t st) {
 }
 
 static int prepare_path_args(sr_handle* h, PathArgs& a, int width, int height, uint64_t seed, uint32_t stream,
-                             int first_block, int n_blocks, int max_depth, int nee_tree = 0) {
+                             int first_block, int n_blocks, int max_depth, int flags = 0) {
   hipStream_t st = h->st;
   int rc = ensure_sum(h, width, height, st);
   if (rc) return rc;
   fill_job(h, h->job_host, width, height, seed, stream, first_block, n_blocks, max_depth);
-  h->job_host.nee_tree = (nee_tree && h->n_lnodes > 0) ? 1 : 0;
+  h->job_host.nee_tree = ((flags & SR_RENDER_LIGHT_TREE) && h->n_lnodes > 0) ? 1 : 0;
+  h->job_host.sss = (flags & SR_RENDER_SUBSURFACE) ? 1 : 0;
   HIPCHK(hipMemcpyAsync(h->jobs.p, &h->job_host, sizeof(Job), hipMemcpyHostToDevice, st));
   if (reset_work(h->work.p, 1, st)) return fail(SR_EHIP, "reset launch failed");
   HIPCHK(hipMemsetAsync(h->overflow.p, 0, 4, st));
@@ -702,8 +704,7 @@ int sr_render(sr_handle* h, int width, int height, uint64_t seed, uint32_t strea
   HIPCHK(hipSetDevice(h->device));
   if (n_blocks == 0) return 0;
   PathArgs a;
-  if ((rc = prepare_path_args(h, a, width, height, seed, stream, first_block, n_blocks, max_depth,
-                              (flags & SR_RENDER_LIGHT_TREE) ? 1 : 0)))
+  if ((rc = prepare_path_args(h, a, width, height, seed, stream, first_block, n_blocks, max_depth, flags)))
     return rc;
   hipStream_t st = h->st;
   const bool sss = (flags & SR_RENDER_SUBSURFACE) != 0;
@@ -892,16 +893,17 @@ int sr_batch_destroy(sr_batch* b) {
 }
 
 int sr_batch_submit(sr_batch* b, sr_handle* h, int width, int height, uint64_t seed, uint32_t stream,
-                    int first_block, int n_blocks, int max_depth, float* dev_accum) {
+                    int first_block, int n_blocks, int max_depth, int flags, float* dev_accum) {
   g_err.clear();
   if (!b) return fail(SR_EINVAL, "null batch");
+  if (flags & ~(SR_RENDER_LIGHT_TREE | SR_RENDER_SUBSURFACE)) return fail(SR_EINVAL, "unknown render flags 0x%x", flags);
   int rc = check_render_args(h, width, height, first_block, n_blocks, max_depth, stream);
   if (rc) return rc;
   if (!dev_accum) return fail(SR_EINVAL, "dev_accum is NULL");
   if (h->device != b->device) return fail(SR_EINVAL, "scene handle is on another device");
   for (const auto& sp : b->specs)
     if (sp.h == h) return fail(SR_EINVAL, "a scene handle may appear once per batch (its block-sum buffer)");
-  b->specs.push_back({h, width, height, first_block, n_blocks, max_depth, seed, stream, dev_accum});
+  b->specs.push_back({h, width, height, first_block, n_blocks, max_depth, seed, stream, dev_accum, flags});
   b->dirty = true;
   return 0;
 }
@@ -927,6 +929,12 @@ int sr_batch_config(sr_batch* b, int block, int grid_pct, int chunk) {
   return 0;
 }
 
+static bool batch_walks(const sr_batch* b) {
+  for (const auto& sp : b->specs)
+    if (sp.flags & SR_RENDER_SUBSURFACE) return true;
+  return false;
+}
+
 static int batch_enqueue(sr_batch* b, int grid) {
   const int n = (int)b->specs.size();
   hipStream_t st = b->st;
@@ -944,7 +952,8 @@ static int batch_enqueue(sr_batch* b, int grid) {
   a.chunk = b->chunk;
   a.adj.rough_mat = a.adj.g_mat = a.adj.sig_mat = -2;
   a.adj.rough_idx = a.adj.g_idx = a.adj.sig_off = -1;
-  if (sr_launch_path(a, grid, b->block, st)) return fail(SR_EHIP, "path kernel launch failed");
+  const int lrc = batch_walks(b) ? sr_launch_path_sss(a, grid, b->block, st) : sr_launch_path(a, grid, b->block, st);
+  if (lrc) return fail(SR_EHIP, "path kernel launch failed");
   for (int j = 0; j < n; ++j) {
     const auto& sp = b->specs[j];
     const size_t cnt = (size_t)sp.W * sp.H * sp.h->B;
@@ -969,6 +978,8 @@ int sr_batch_launch(sr_batch* b, int flags) {
       int rc = ensure_sum(sp.h, sp.W, sp.H, b->st);
       if (rc) return rc;
       fill_job(sp.h, b->host_jobs[j], sp.W, sp.H, sp.seed, sp.stream, sp.first, sp.nb, sp.depth);
+      b->host_jobs[j].nee_tree = ((sp.flags & SR_RENDER_LIGHT_TREE) && sp.h->n_lnodes > 0) ? 1 : 0;
+      b->host_jobs[j].sss = (sp.flags & SR_RENDER_SUBSURFACE) ? 1 : 0;
     }
     int rc = b->jobs.upload(b->host_jobs.data(), (size_t)n, b->st);
     if (!rc) rc = b->work.ensure(work_bytes(n));
@@ -977,7 +988,8 @@ int sr_batch_launch(sr_batch* b, int flags) {
     b->dirty = false;
   }
   int per = 0;
-  if (sr_path_grid_bs(b->block, &per) != 0) return fail(SR_EHIP, "occupancy query failed");
+  if ((batch_walks(b) ? sr_path_sss_grid_bs(b->block, &per) : sr_path_grid_bs(b->block, &per)) != 0)
+    return fail(SR_EHIP, "occupancy query failed");
   int grid = std::max(1, per * b->grid_pct / 100);
   HIPCHK(hipEventRecord(b->ev0, b->st));
   if (flags & SR_BATCH_GRAPH) {

@@ -65,6 +65,7 @@ struct Job {
   int W, H, depth, first_block, n_blocks, tiled;
   uint32_t seed_lo, seed_hi, stream;
   int nee_tree;      // 1: NEE picks lights through the light tree (SR_RENDER_LIGHT_TREE)
+  int sss;           // 1: subsurface random walk (SR_RENDER_SUBSURFACE; the walk kernel only)
 };
 
 struct PathArgs {
@@ -94,6 +95,7 @@ size_t sr_path_lds_bytes(int block, bool adjoint);
 int sr_launch_path(const PathArgs& a, int grid, int block, hipStream_t st);
 int sr_launch_adjoint(const PathArgs& a, int grid, int block, hipStream_t st);
 int sr_path_sss_grid(int* grid_out);
+int sr_path_sss_grid_bs(int block, int* grid_out);
 int sr_launch_path_sss(const PathArgs& a, int grid, int block, hipStream_t st);
 int sr_launch_fill_u32(uint32_t* p, size_t n_words, uint32_t v, hipStream_t st);
 int sr_launch_finalize(float* mean, float* sum, size_t n, int first, int nb, hipStream_t st);

@@ -270,7 +270,7 @@ __device__ __forceinline__ void scatter_surface(Lane& L, const Ctx<ADJ>& c, floa
   } else {
     float absorb = fdiv(4.0f * SR_PI * kt, s.lam_m[ab]);
     float dte = fdiv(-logf(-rs.u[3] + 1.0f + 1e-7f), sm.ss + absorb + 1e-6f);
-    if (SSS && !sm.tv && sm.ss > 0.0f) {
+    if (SSS && a.sss && !sm.tv && sm.ss > 0.0f) {
       // random-walk subsurface scattering (the TODO of main.py:1066) in
       // place of the single-scatter Oren-Nayar lobe: enter along a cosine-
       // distributed direction into the surface; walk_step does the rest
@@ -867,7 +867,7 @@ __device__ __forceinline__ void on_shadow(Lane& L, const Ctx<ADJ>& c, const HitD
       pdf = cosv * SR_INV_PI;
     } else if (L.nee_surf != -1) {
       cosv = tmax(0.0f, dot(L.esn, L.sdir));
-      evaluate_bsdf(s, L.vol, L.nee_surf, -L.dir, L.sdir, L.esn, ab, bsdf, pdf, SSS);
+      evaluate_bsdf(s, L.vol, L.nee_surf, -L.dir, L.sdir, L.esn, ab, bsdf, pdf, SSS && c.a.sss);
     } else {
       cosv = tmax(0.0f, dot(L.dir, L.sdir));
       bsdf = hg_eval(cosv, get_mat(s, L.vol, ab).g);
@@ -1129,9 +1129,10 @@ int sr_launch_path(const PathArgs& a, int grid, int block, hipStream_t st) {
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int sr_path_sss_grid(int* grid_out) {
-  return grid_for((const void*)sr_path_sss_kernel, 256, sr_path_lds_bytes(256, false), grid_out);
+int sr_path_sss_grid_bs(int block, int* grid_out) {
+  return grid_for((const void*)sr_path_sss_kernel, block, sr_path_lds_bytes(block, false), grid_out);
 }
+int sr_path_sss_grid(int* grid_out) { return sr_path_sss_grid_bs(256, grid_out); }
 
 int sr_launch_path_sss(const PathArgs& a, int grid, int block, hipStream_t st) {
   hipLaunchKernelGGL(sr_path_sss_kernel, dim3(grid), dim3(block), sr_path_lds_bytes(block, false), st, a);

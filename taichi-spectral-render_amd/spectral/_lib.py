@@ -55,7 +55,7 @@ SIGNATURES = {
     "sr_batch_create": ([ctypes.POINTER(_vp)], ctypes.c_int),
     "sr_batch_destroy": ([_vp], ctypes.c_int),
     "sr_batch_submit": ([_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int,
-                         ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
+                         ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
     "sr_batch_clear": ([_vp], ctypes.c_int),
     "sr_batch_config": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "sr_batch_launch": ([_vp, ctypes.c_int], ctypes.c_int),

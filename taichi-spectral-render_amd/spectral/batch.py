@@ -52,13 +52,15 @@ class Batch:
         except Exception:
             pass
 
-    def submit(self, renderer: Renderer, n_blocks, max_depth, seed=0, stream=0):
+    def submit(self, renderer: Renderer, n_blocks, max_depth, seed=0, stream=0, light_tree=False, subsurface=False):
         """Add a job: `n_blocks` spectral blocks of `renderer`'s scene into its
-        accumulator (continuing from renderer.blocks_done)."""
+        accumulator (continuing from renderer.blocks_done), with the render
+        modes of Renderer.render."""
         if renderer.accum is None:
             raise RuntimeError("renderer has no resolution")
+        flags = (1 if light_tree else 0) | (2 if subsurface else 0)
         _lib.call("sr_batch_submit", self._b, renderer._h, renderer.width, renderer.height, ctypes.c_uint64(seed),
-                  ctypes.c_uint32(stream), renderer.blocks_done, int(n_blocks), int(max_depth),
+                  ctypes.c_uint32(stream), renderer.blocks_done, int(n_blocks), int(max_depth), flags,
                   _lib.ptr(renderer.accum))
         self.jobs.append((renderer, int(n_blocks), int(max_depth), int(seed), int(stream)))
 

@@ -69,3 +69,35 @@ def test_batch_config_and_tuner(gpu):
         b.submit(bat[0][1], 1, 4, 0)  # a handle may appear once per batch
     out = b.run()
     assert all(s > 0 for s in out["segments"])
+
+
+def test_batch_with_render_modes(gpu):
+    """A light-tree job, a subsurface job and a plain job in one launch (the
+    walk kernel, walking only in the job that asks for it) reproduce their
+    own sr_render calls with the same flags."""
+    from spectral.batch import Batch
+    from spectral.render import Renderer
+
+    cfgs = [(configs.lamps(width=16, height=16), {"light_tree": True}),
+            (configs.subsurface_slab(width=16, height=16), {"subsurface": True}),
+            (configs.box_icosphere(width=16, height=16), {})]
+
+    def make():
+        out = []
+        for c, kw in cfgs:
+            f = flatten_scene(c.scene, c.bands, c.wl_min, c.wl_max)
+            out.append((c, kw, Renderer(f, c.bands, c.wl_min, c.wl_max, width=c.width, height=c.height, device=gpu)))
+        return out
+
+    ref, segs = [], []
+    for c, kw, r in make():
+        ref.append(r.render(n_blocks=4, max_depth=c.depth, seed=c.seed, **kw).cpu().numpy())
+        segs.append(r.stats()["segments"])
+    bat = make()
+    b = Batch()
+    for c, kw, r in bat:
+        b.submit(r, 4, c.depth, c.seed, **kw)
+    out = b.run()
+    for k, (c, kw, r) in enumerate(bat):
+        assert np.allclose(r.accum.cpu().numpy(), ref[k], rtol=1e-5, atol=1e-6), c.name
+    assert list(out["segments"]) == segs
