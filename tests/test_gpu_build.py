@@ -3,7 +3,8 @@
 The device builder reproduces the host builder's tree, so the test is
 byte identity of every compiled array (nodes, triangle records, shading
 records, light tables) read back through sr_scene_array, on the 1292-face
-box scene, the 1,048,588-face heightfield, 16 of the 64 batch scenes, and
+box scene, the 1,048,588- and 4,194,316-face heightfields, 16 of the 64
+batch scenes, and
 scenes made to reach the builder's other paths: coincident centroids (every
 split an object median with ties, in both the chunked and the one-wave
 kernels), faces at x = -2^-k (SAH splits that peel off a few faces until
@@ -105,6 +106,16 @@ def test_box_render_matches_host_build(box):
 
 def test_heightfield_1m_identical_to_host_build(gpu):
     cfg = configs.heightfield_1m()
+    flat = flatten_scene(cfg.scene, cfg.bands, cfg.wl_min, cfg.wl_max)
+    host, dev = _pair(flat, gpu, cfg.bands, cfg.wl_min, cfg.wl_max)
+    assert dev.scene_info()["bvh4_depth"] <= 13
+    _assert_same_scene(host, dev)
+
+
+def test_heightfield_4m_identical_to_host_build(gpu):
+    """4,194,316 faces (2048 x 1024 cells): the level kernels at four times
+    the headline size, a tree at the 4-wide depth bound."""
+    cfg = configs.heightfield_1m(nx=2048, nz=1024, width=64, height=64)
     flat = flatten_scene(cfg.scene, cfg.bands, cfg.wl_min, cfg.wl_max)
     host, dev = _pair(flat, gpu, cfg.bands, cfg.wl_min, cfg.wl_max)
     assert dev.scene_info()["bvh4_depth"] <= 13
