@@ -271,7 +271,8 @@ def bench_build(reps=3, log=print):
 def bench_rows(reps=5, log=print):
     """Throughput of the rows around the path: the display path on
     heightfield_1m at 1024x1024 (normals view: one closed-hit trace per
-    pixel; RGBA8 conversion: 12 B read + 4 B written per pixel), NEE through
+    pixel; RGBA8 conversion: 12 B read + 4 B written per pixel; CMF resolve
+    of the 31-band accumulator: 124 B read + 12 B written per pixel), NEE through
     the light tree against the area CDF on 128 lamps (256x256, depth 4), and
     the subsurface walk on the closed slab (256x256, w = 0.9)."""
     import numpy as np
@@ -304,7 +305,16 @@ def bench_rows(reps=5, log=print):
     npx = 1024 * 1024
     out["normals_view"] = {"pixels": npx, "ms": round(ms_n, 3), "Mpixels_per_s": round(npx / ms_n / 1e3, 1)}
     gbs = npx * 16 / ms_c / 1e6
-    out["rgba8"] = {"pixels": npx, "ms": round(ms_c, 4), "GB_per_s": round(gbs, 1), "frac_of_8TBps": round(gbs / 8000, 3)}
+    out["rgba8"] = {"pixels": npx, "ms": round(ms_c, 4), "GB_per_s": round(gbs, 1), "frac_of_8TBps": round(gbs / 8000, 3),
+                    "note": "16 MB per frame: launch-bound at this size"}
+    # CMF resolve of a full accumulator (color_rays): reads W x H x B f32, writes W x H x 3
+    from spectral.data import srgb_colour_system
+    cs = srgb_colour_system(cfg.wl_min, cfg.wl_max, cfg.bands)
+    r.render(n_blocks=1, max_depth=cfg.depth, seed=cfg.seed)
+    ms_r = timed(lambda: [r.color_rays(cs) for _ in range(20)]) / 20  # back to back: amortises the host call
+    rb = npx * (cfg.bands + 3) * 4
+    out["resolve"] = {"bytes": rb, "ms": round(ms_r, 4), "GB_per_s": round(rb / ms_r / 1e6, 1),
+                      "frac_of_8TBps": round(rb / ms_r / 1e6 / 8000, 3)}
     del r
     for name, cfgf, kw in (("light_tree", configs.lamps, {}), ("subsurface", configs.subsurface_slab, {})):
         c = cfgf(width=256, height=256)

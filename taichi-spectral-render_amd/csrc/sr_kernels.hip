@@ -45,14 +45,7 @@ __global__ void sr_fill_kernel(float* __restrict__ p, size_t n, float v) {
 // ====================================================================== resolve
 // color_rays, main.py:1609-1629: CMF projection, XYZ -> RGB, Reinhard,
 // gamma, nearest upsample from the render grid to the display grid.
-__global__ void sr_resolve_kernel(const float* __restrict__ acc, int W, int H, int B, ResolveParams p,
-                                  float* __restrict__ out, int OW, int OH) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= OW * OH) return;
-  const int i = idx / OH, j = idx % OH;
-  const int ri = (int)((float)i / (float)OW * (float)W);
-  const int rj = (int)((float)j / (float)OH * (float)H);
-  const float* a = acc + ((size_t)ri * H + rj) * B;
+__device__ __forceinline__ void resolve_pixel(const float* a, int B, const ResolveParams& p, float* o) {
   float x = 0.f, y = 0.f, z = 0.f;
   for (int k = 0; k < B; ++k) {
     float v = a[k];
@@ -69,8 +62,37 @@ __global__ void sr_resolve_kernel(const float* __restrict__ acc, int W, int H, i
     float m = e / (e + 1.0f);
     rgb[r] = powf(m, 1.0f / p.gamma);
   }
-  float* o = out + (size_t)idx * 3;
   o[0] = rgb[0]; o[1] = rgb[1]; o[2] = rgb[2];
+}
+
+// Display grid != render grid: one thread per display pixel.
+__global__ void sr_resolve_kernel(const float* __restrict__ acc, int W, int H, int B, ResolveParams p,
+                                  float* __restrict__ out, int OW, int OH) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= OW * OH) return;
+  const int i = idx / OH, j = idx % OH;
+  const int ri = (int)((float)i / (float)OW * (float)W);
+  const int rj = (int)((float)j / (float)OH * (float)H);
+  resolve_pixel(acc + ((size_t)ri * H + rj) * B, B, p, out + (size_t)idx * 3);
+}
+
+// Display grid == render grid: the 256 pixels of a workgroup own a
+// contiguous slice of the [W][H][B] accumulator; it is staged in LDS with
+// 16-byte loads (coalesced), then each thread reads its pixel's B values
+// there (B odd or not, consecutive pixels start B words apart).
+__global__ __launch_bounds__(256) void sr_resolve_tile_kernel(const float* __restrict__ acc, int npix, int B,
+                                                              ResolveParams p, float* __restrict__ out) {
+  extern __shared__ float tile[];
+  const int p0 = blockIdx.x * 256;
+  const int np = min(256, npix - p0);
+  const size_t base = (size_t)p0 * B;  // a multiple of 256 B floats: 16-byte aligned
+  const int nf = np * B;
+  const float4* a4 = reinterpret_cast<const float4*>(acc + base);
+  float4* t4 = reinterpret_cast<float4*>(tile);
+  for (int t = threadIdx.x; t < nf / 4; t += 256) t4[t] = a4[t];
+  for (int t = (nf & ~3) + threadIdx.x; t < nf; t += 256) tile[t] = acc[base + t];
+  __syncthreads();
+  if ((int)threadIdx.x < np) resolve_pixel(tile + threadIdx.x * B, B, p, out + (size_t)(p0 + threadIdx.x) * 3);
 }
 
 // ====================================================================== display
@@ -162,6 +184,11 @@ int sr_launch_resolve(const float* acc, int W, int H, int B, const ResolveParams
                       hipStream_t st) {
   int n = OW * OH;
   if (n <= 0) return 0;
-  hipLaunchKernelGGL(sr_resolve_kernel, dim3((n + 255) / 256), dim3(256), 0, st, acc, W, H, B, p, out, OW, OH);
+  if (OW == W && OH == H && ((uintptr_t)acc & 15u) == 0) {
+    hipLaunchKernelGGL(sr_resolve_tile_kernel, dim3((n + 255) / 256), dim3(256), (size_t)256 * B * sizeof(float), st,
+                       acc, n, B, p, out);
+  } else {
+    hipLaunchKernelGGL(sr_resolve_kernel, dim3((n + 255) / 256), dim3(256), 0, st, acc, W, H, B, p, out, OW, OH);
+  }
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

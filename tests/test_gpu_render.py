@@ -138,23 +138,24 @@ def test_running_mean_blocks_split(box_small, gpu):
     assert np.allclose(a.cpu().numpy(), r.accum.cpu().numpy(), rtol=1e-5, atol=1e-7)
 
 
-def test_resolve_matches_color_rays_closed_form(box_small, gpu):
+@pytest.mark.parametrize("W,H,OW,OH", [(10, 6, 20, 12), (37, 29, 37, 29)])  # upsampled; tiled (partial tile)
+def test_resolve_matches_color_rays_closed_form(box_small, gpu, W, H, OW, OH):
     import torch
 
     from spectral.data import srgb_colour_system
 
     cfg, flat, _ = box_small
-    r = _renderer(flat, cfg, 10, 6, gpu)
+    r = _renderer(flat, cfg, W, H, gpu)
     rng = np.random.default_rng(0)
-    acc = rng.uniform(0, 2, (10, 6, cfg.bands)).astype(np.float32)
+    acc = rng.uniform(0, 2, (W, H, cfg.bands)).astype(np.float32)
     r.accum.copy_(torch.from_numpy(acc))
     cs = srgb_colour_system(cfg.wl_min, cfg.wl_max, cfg.bands)
-    out = r.color_rays(cs, exposure=1.0, gamma=2.0, out_width=20, out_height=12).cpu().numpy()
+    out = r.color_rays(cs, exposure=1.0, gamma=2.0, out_width=OW, out_height=OH).cpu().numpy()
     # color_rays (main.py:1609-1629) in float64
     step = (cfg.wl_max - cfg.wl_min) / (cfg.bands - 1)
     norm = np.sum(cs.cmf[:, 1]) * step
-    ri = (np.arange(20) / 20 * 10).astype(int)
-    rj = (np.arange(12) / 12 * 6).astype(int)
+    ri = (np.arange(OW) / OW * W).astype(int)
+    rj = (np.arange(OH) / OH * H).astype(int)
     a = acc[ri][:, rj].astype(np.float64)
     xyz = a @ cs.cmf * (step / norm)
     rgb = xyz @ cs.T.T
