@@ -273,8 +273,10 @@ def bench_rows(reps=5, log=print):
     heightfield_1m at 1024x1024 (normals view: one closed-hit trace per
     pixel; RGBA8 conversion: 12 B read + 4 B written per pixel; CMF resolve
     of the 31-band accumulator: 124 B read + 12 B written per pixel), NEE through
-    the light tree against the area CDF on 128 lamps (256x256, depth 4), and
-    the subsurface walk on the closed slab (256x256, w = 0.9)."""
+    the light tree against the area CDF (time, pixel variance over 8 blocks and
+    efficiency = 1 / (variance x time)) on 128 lamps over 3.5 units and on
+    1024 lamps over 32 units, and the subsurface walk on the closed slab
+    (w = 0.9), all at 1024x1024 so that one block fills the persistent grid."""
     import numpy as np
     import torch
 
@@ -316,25 +318,44 @@ def bench_rows(reps=5, log=print):
     out["resolve"] = {"bytes": rb, "ms": round(ms_r, 4), "GB_per_s": round(rb / ms_r / 1e6, 1),
                       "frac_of_8TBps": round(rb / ms_r / 1e6 / 8000, 3)}
     del r
-    for name, cfgf, kw in (("light_tree", configs.lamps, {}), ("subsurface", configs.subsurface_slab, {})):
-        c = cfgf(width=256, height=256)
+    rows = (("light_tree", configs.lamps(width=1024, height=1024)),
+            ("light_tree_sparse", configs.lamps(width=1024, height=1024, n=32, extent=32.0)),
+            ("subsurface", configs.subsurface_slab(width=1024, height=1024)))
+    for name, c in rows:
         f = flatten_scene(c.scene, c.bands, c.wl_min, c.wl_max)
-        rr = Renderer(f, c.bands, c.wl_min, c.wl_max, width=256, height=256, device="cuda:0")
-        modes = (("cdf", {}), ("tree", {"light_tree": True})) if name == "light_tree" else \
-            (("walk", {"subsurface": True}),)
-        res = {"workload": f"{c.name}: {f.n_tris} tris, {c.bands} bands, 256x256, depth {c.depth}"}
+        rr = Renderer(f, c.bands, c.wl_min, c.wl_max, width=1024, height=1024, device="cuda:0")
+        light = name.startswith("light_tree")
+        modes = (("cdf", {}), ("tree", {"light_tree": True})) if light else (("walk", {"subsurface": True}),)
+        res = {"workload": f"{c.name}: {f.n_tris} tris, {c.bands} bands, 1024x1024, depth {c.depth}"}
         for mode, flags in modes:
             rr.render(n_blocks=1, max_depth=c.depth, seed=c.seed, **flags)
             rr.sync()
             ms = []
-            for k in range(reps):
-                rr.clear_pixel_space()
-                rr.render(n_blocks=1, max_depth=c.depth, seed=c.seed + 1 + k, **flags)
+            s1 = s2 = None
+            n_var = 8 if light else reps
+            for k in range(n_var):
+                # block k of one seed: the accumulator holds block / (k + 1)
+                rr.accum.zero_()
+                rr.blocks_done = k
+                rr.render(n_blocks=1, max_depth=c.depth, seed=c.seed, **flags)
                 ms.append(rr.last_kernel_ms())
+                if light:
+                    x = rr.accum.double() * (k + 1)
+                    s1 = x if s1 is None else s1 + x
+                    s2 = x * x if s2 is None else s2 + x * x
             st = rr.stats()
             m = float(np.median(ms))
             res[mode] = {"kernel_ms_per_block": round(m, 3), "Mseg_per_s": round(st["segments"] / m / 1e3, 1),
                          "traversals_per_segment": round(st["traversals"] / max(st["segments"], 1), 3)}
+            if light:
+                # per pixel-band sample variance over the blocks, summed over the image
+                res[mode]["variance"] = float(((s2 - s1 * s1 / n_var) / (n_var - 1)).sum())
+        if light:
+            # efficiency = 1 / (variance x time): the tree's gain over the CDF pick
+            cdf, tree = res["cdf"], res["tree"]
+            res["tree_variance_ratio"] = round(tree["variance"] / cdf["variance"], 4)
+            res["tree_efficiency_gain"] = round(cdf["variance"] * cdf["kernel_ms_per_block"]
+                                                / (tree["variance"] * tree["kernel_ms_per_block"]), 3)
         out[name] = res
         del rr
     return out

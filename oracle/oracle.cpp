@@ -22,6 +22,12 @@
 //   cast_rays                   light-visible pixels = closed form,
 //                               zero-light scene = 0, escaped paths = 0
 //   EEM categorical band pick   row-normalised pmf histogram
+//   light-tree NEE (optional)   the build's own extension (main.py:541 is a
+//                               TODO): the tree arrays are given, the
+//                               descent and pdf walk are restated from their
+//                               definition in IEEE f32; pinned by the
+//                               structural and expectation tests of the GPU
+//                               tree mode (tests/test_gpu_lighttree.py)
 //
 // Deliberate, documented deviations shared with the HIP path:
 //   * light CDF last entry forced to 1.0 (reference main.py:619-621 leaves
@@ -147,6 +153,16 @@ struct Tri {
   int mat;
   float area;
 };
+struct LNode {  // one light-tree node, 64 bytes as the device stores it
+  float lo[3], cos_o, hi[3];
+  int left;  // inner: first child (the second is left + 1); leaf: ~slot
+  float axis[3];
+  int right;  // inner: left + 1; leaf: -1
+  int parent, face;
+  float sin_o;
+  int pad;
+};
+static_assert(sizeof(LNode) == 64, "LNode mirrors the 64-byte device node");
 struct Mat {  // MaterialSample, main.py:277-289
   float n, k, r, ss, em, g;
   int tv, se, eem;
@@ -173,6 +189,13 @@ struct Scene {
   std::vector<std::vector<int>> light_idx;    // [B][n]
   V3 cam_o, cam_lookat, cam_up, cam_u, cam_v, cam_w;
   bool brute = false;  // trace by brute force instead of the reference BVH
+  // optional light tree (many-light NEE): nodes in sibling pairs (root 0,
+  // node 1 unused, the children of an inner node at an even index and the
+  // next), power [B][n], leaf of each face (-1: not a light slot)
+  bool tree = false;
+  std::vector<LNode> lt_nodes;
+  std::vector<float> lt_power;
+  std::vector<int> lt_leaf;
 };
 
 static inline Mat get_mat(const Scene& s, int m, int b) {  // main.py:980-996
@@ -863,7 +886,79 @@ static float shadow_transmittance(const Scene& s, V3 origin, V3 dir, float max_d
   return total;
 }
 
-// sample_direct_light, main.py:1296-1358
+// Light-tree pick.  A node's importance for a point x is its band power P
+// (emitting area) times a bound on the emitters' cosine towards x over the
+// squared distance to the node centre (floored at 1e-3 r^2 for points near
+// the centre of a large node):  with theta the angle between the node's
+// cone axis and x - centre, theta_o the cone half angle and theta_u the
+// half angle of the bounding sphere seen from x, the bound is
+// cos(max(0, theta - theta_o - theta_u)), 1 inside the sphere, and 0 when
+// that angle reaches pi/2 (every emitter faces away).  Children are picked
+// in proportion to their importance, with one uniform rescaled per level.
+static float lt_importance(const LNode& nd, float P, V3 x) {
+  if (!(P > 0.0f)) return 0.0f;
+  V3 lo = v3(nd.lo[0], nd.lo[1], nd.lo[2]), hi = v3(nd.hi[0], nd.hi[1], nd.hi[2]);
+  V3 dv = x - (lo + hi) * 0.5f;
+  V3 ext = hi - lo;
+  float r2 = 0.25f * dot(ext, ext);
+  float d2 = dot(dv, dv);
+  float cb = 1.0f;
+  if (d2 > r2) {
+    float d = std::sqrt(d2);
+    float th = std::acos(clampf(dot(v3(nd.axis[0], nd.axis[1], nd.axis[2]), dv) / d, -1.0f, 1.0f));
+    float th_o = std::atan2(nd.sin_o, nd.cos_o);
+    float th_u = std::asin(tmin(1.0f, std::sqrt(r2) / d));
+    float a = tmax(0.0f, th - th_o - th_u);
+    if (a >= 0.5f * PI_F) return 0.0f;
+    cb = std::cos(a);
+  }
+  return P * cb / tmax(d2, 1e-3f * r2 + 1e-12f);
+}
+
+static int lt_sample(const Scene& s, int band, V3 x, float u, float& p) {
+  const int n = (int)s.lt_nodes.size();
+  p = 1.0f;
+  if (s.lt_nodes[0].right < 0) return 0;
+  int f = s.lt_nodes[0].left;
+  for (;;) {
+    float il = lt_importance(s.lt_nodes[f], s.lt_power[(size_t)band * n + f], x);
+    float ir = lt_importance(s.lt_nodes[f + 1], s.lt_power[(size_t)band * n + f + 1], x);
+    float sum = il + ir;
+    if (!(sum > 0.0f)) return -1;
+    float pl = il / sum;
+    int c;
+    if (u < pl || !(ir > 0.0f)) {
+      u = tmin(u / pl, 0x1.fffffep-1f);
+      p *= pl;
+      c = f;
+    } else {
+      float pr = ir / sum;
+      u = tmin((u - pl) / pr, 0x1.fffffep-1f);
+      p *= pr;
+      c = f + 1;
+    }
+    if (s.lt_nodes[c].right < 0) return c;
+    f = s.lt_nodes[c].left;
+  }
+}
+
+static float lt_pdf(const Scene& s, int band, V3 x, int leaf) {
+  const int n = (int)s.lt_nodes.size();
+  float p = 1.0f;
+  for (int c = leaf; c != 0; c = s.lt_nodes[c].parent) {
+    int f = c & ~1;
+    float il = lt_importance(s.lt_nodes[f], s.lt_power[(size_t)band * n + f], x);
+    float ir = lt_importance(s.lt_nodes[f + 1], s.lt_power[(size_t)band * n + f + 1], x);
+    float sum = il + ir;
+    if (!(sum > 0.0f)) return 0.0f;
+    p *= (c == f ? il : ir) / sum;
+  }
+  return p;
+}
+
+// sample_direct_light, main.py:1296-1358 (with the light tree, the picked
+// light's area over its pick probability takes the place of the band's
+// total emitting area in the estimator and the MIS pdf)
 static NEE sample_direct_light(const Scene& s, const Ray& ray, V3 normVec, int surf_id, Rng& rng, Counters& cnt) {
   NEE res;
   V3 p = ray.position;
@@ -873,8 +968,17 @@ static NEE sample_direct_light(const Scene& s, const Ray& ray, V3 normVec, int s
     rng.slot(SLOT_NEE);
     float r = rng.next();
     int lt_idx = 0;
-    for (int i = 0; i < nl; ++i) {
-      if (r < s.light_cdf[wl][i]) { lt_idx = s.light_idx[wl][i]; break; }
+    float area = s.light_area[wl];
+    if (s.tree) {
+      float pk;
+      int leaf = lt_sample(s, wl, p, r, pk);
+      if (leaf < 0) return res;  // every light of the band faces away
+      lt_idx = s.lt_nodes[leaf].face;
+      area = s.tris[lt_idx].area / pk;
+    } else {
+      for (int i = 0; i < nl; ++i) {
+        if (r < s.light_cdf[wl][i]) { lt_idx = s.light_idx[wl][i]; break; }
+      }
     }
     const Tri& lt = s.tris[lt_idx];
     float bx = rng.next(), by = rng.next();
@@ -897,9 +1001,9 @@ static NEE sample_direct_light(const Scene& s, const Ray& ray, V3 normVec, int s
         bsdf = evaluate_henyey_greenstein(cos_theta, get_mat(s, ray.vol_mat_id, wl).g);
         pdf = bsdf;
       }
-      res.contribution_factor = bsdf * cos_theta * tr * gterm * s.light_area[wl];
+      res.contribution_factor = bsdf * cos_theta * tr * gterm * area;
       res.emission = get_mat(s, lt.mat, ray.transport).em;
-      float pdf_light = d2 / (tmax(1e-6f, cl) * s.light_area[wl]);
+      float pdf_light = d2 / (tmax(1e-6f, cl) * area);
       res.weight = (pdf_light * pdf_light) / (pdf_light * pdf_light + pdf * pdf + 1e-6f);
     }
   }
@@ -927,6 +1031,7 @@ static void cast_ray_path(const Scene& s, int W, int H, int i, int j, int band, 
   ray.has_terminated = 0;
   ray.surface_pdf = 1.0f;
   int active_field = band;  // active_wavelength_idx[i, j]
+  V3 nee_pos = ray.position;  // where the last NEE was taken (light-tree MIS pdf)
   for (int k = 0; k < depth; ++k) {
     if (ray.has_terminated) break;
     rng.seg = (uint32_t)k;
@@ -947,13 +1052,24 @@ static void cast_ray_path(const Scene& s, int W, int H, int i, int j, int band, 
         const Tri& lt = s.tris[ev.tri_idx];
         float d2 = ev.travel_distance * ev.travel_distance;
         float cl = tmax(0.0f, dot(lt.normal, -ray.direction));
-        float pdf_light = (d2 * tmin(1.0f, (float)ray.path_length)) / (cl * s.light_area[ray.active] + 1e-6f);
+        float lden = cl * s.light_area[ray.active] + 1e-6f;
+        if (s.tree && cl > 0.0f && ray.path_length > 0) {
+          // the tree's pick probability of this face from the previous NEE
+          // vertex; a face the band's NEE cannot pick keeps the total area
+          int leaf = s.lt_leaf[ev.tri_idx];
+          if (leaf >= 0 && s.lt_power[(size_t)ray.active * s.lt_nodes.size() + leaf] > 0.0f) {
+            float pk = lt_pdf(s, ray.active, nee_pos, leaf);
+            if (pk > 0.0f) lden = cl * (lt.area / pk) + 1e-6f;
+          }
+        }
+        float pdf_light = (d2 * tmin(1.0f, (float)ray.path_length)) / lden;
         float w = (ray.surface_pdf * ray.surface_pdf) /
                   (pdf_light * pdf_light + ray.surface_pdf * ray.surface_pdf + 1e-6f);
         indirect = ray.throughput * sm.em * w;
         ray.has_terminated = 1;
       } else {
         NEE ns = sample_direct_light(s, ray, ev.shading_normal, ev.hit_mat_id, rng, cnt);
+        nee_pos = ray.position;
         direct = ray.throughput * ns.emission * ns.contribution_factor * ns.weight;
         ray.path_length += 1;
         SurfaceEvent se = scatter_surface(s, ray, ev, rng);
@@ -967,6 +1083,7 @@ static void cast_ray_path(const Scene& s, int W, int H, int i, int j, int band, 
       }
     } else {
       NEE ns = sample_direct_light(s, ray, ev.shading_normal, ev.hit_mat_id, rng, cnt);
+      nee_pos = ray.position;
       direct = ray.throughput * ns.emission * ns.contribution_factor * ns.weight;
       if (ev.hit_action == 2) {
         ray.direction = ev.direction;
@@ -1074,6 +1191,26 @@ void* or_scene_create(int B, float wl_min, float wl_max, const float* tris_v, co
 }
 
 void or_scene_destroy(void* p) { delete (Scene*)p; }
+
+// Light tree for NEE (enable = 0 returns to the area-CDF pick): n nodes of
+// 64 bytes in sibling pairs, power [B][n] f32, keys {face, leaf}[n_keys].
+int or_scene_light_tree(void* p, int enable, int n, const void* nodes, const float* power, int n_keys,
+                        const int* keys) {
+  Scene* s = (Scene*)p;
+  s->tree = false;
+  if (!enable) return 0;
+  if (n < 2 || n % 2 != 0) return -1;
+  s->lt_nodes.resize(n);
+  std::memcpy(s->lt_nodes.data(), nodes, (size_t)n * sizeof(LNode));
+  s->lt_power.assign(power, power + (size_t)s->B * n);
+  s->lt_leaf.assign(s->tris.size(), -1);
+  for (int k = 0; k < n_keys; ++k) {
+    if (keys[2 * k] < 0 || keys[2 * k] >= (int)s->tris.size() || keys[2 * k + 1] < 0 || keys[2 * k + 1] >= n) return -1;
+    s->lt_leaf[keys[2 * k]] = keys[2 * k + 1];
+  }
+  s->tree = true;
+  return 0;
+}
 
 int or_scene_info(void* p, int* n_nodes, int* max_lights) {
   Scene* s = (Scene*)p;

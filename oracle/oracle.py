@@ -25,6 +25,7 @@ def load():
         lib.or_scene_create.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_float, _vp, _vp, _vp, ctypes.c_int, _vp,
                                         _vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp, ctypes.c_int]
         lib.or_scene_destroy.argtypes = [_vp]
+        lib.or_scene_light_tree.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp]
         lib.or_scene_info.argtypes = [_vp, _vp, _vp]
         lib.or_trace.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]
         lib.or_render.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int,
@@ -76,6 +77,21 @@ class OracleScene:
         if getattr(self, "h", None):
             load().or_scene_destroy(self.h)
             self.h = None
+
+    def light_tree(self, nodes=None, power=None, keys=None):
+        """NEE through a light tree (the device's arrays: nodes as raw
+        64-byte records, power [B][n] f32, keys {face, leaf} int32 pairs);
+        no arguments return to the area-CDF pick."""
+        if nodes is None:
+            load().or_scene_light_tree(self.h, 0, 0, None, None, 0, None)
+            return
+        nodes = np.ascontiguousarray(nodes, np.uint8)
+        power = _f32(power)
+        keys = np.ascontiguousarray(keys, np.int32)
+        n = nodes.nbytes // 64
+        if load().or_scene_light_tree(self.h, 1, n, _p(nodes), _p(power), keys.size // 2, _p(keys)) != 0:
+            raise ValueError("malformed light tree arrays")
+        self._tree = (nodes, power, keys)
 
     def info(self):
         n = np.zeros(1, np.int32)

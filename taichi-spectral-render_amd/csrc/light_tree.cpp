@@ -128,8 +128,9 @@ struct Builder {
 
 // slots: emissive faces in light-slot order; lverts [n][3][4] f32 (xyz_);
 // slot_mat: material of each slot; se [M][B]: surface_emission flags.
-// Writes nodes (preorder, root 0), power [nodes][B] (emitting area per
-// band, f32 of the f64 sums) and keys (face -> leaf, sorted by face).
+// Writes nodes (sibling pairs, root 0, node 1 unused), power [B][nodes]
+// (emitting area per band, f32 of the f64 sums) and keys (face -> leaf,
+// sorted by face).
 int sr_build_light_tree(int n, const float* lverts, const int* slots, const int* slot_mat, const unsigned char* se,
                         int B, std::vector<LightNode>& nodes, std::vector<float>& power, std::vector<LightKey>& keys) {
   nodes.clear();
@@ -175,11 +176,48 @@ int sr_build_light_tree(int n, const float* lverts, const int* slots, const int*
   for (int s = 0; s < n; ++s) bl.idx[s] = s;
   bl.nodes.reserve((size_t)2 * n);
   bl.build(0, n, -1);
-  nodes = bl.nodes;
-  power.resize(bl.npow.size());
-  for (size_t i = 0; i < power.size(); ++i) power[i] = (float)bl.npow[i];
-  for (int i = 0; i < (int)nodes.size(); ++i)
-    if (nodes[i].right < 0) keys.push_back(LightKey{nodes[i].face, i});
+  // Sibling-pair layout: root at 0, an unused node at 1, and the two
+  // children of every inner node at an even index and the next (breadth
+  // first), so one descent step reads both children as one 128-byte line
+  // and their powers in one band row as one 8-byte pair.
+  const int nb = (int)bl.nodes.size();
+  const int nn = nb + 1;
+  std::vector<int> at((size_t)nb, -1), order;
+  order.reserve((size_t)nn);
+  at[0] = 0;
+  order.push_back(0);
+  order.push_back(-1);  // the unused node
+  for (size_t q = 0; q < order.size(); ++q) {
+    const int o = order[q];
+    if (o < 0 || bl.nodes[o].right < 0) continue;
+    at[bl.nodes[o].left] = (int)order.size();
+    order.push_back(bl.nodes[o].left);
+    at[bl.nodes[o].right] = (int)order.size();
+    order.push_back(bl.nodes[o].right);
+  }
+  nodes.assign((size_t)nn, LightNode{});
+  power.assign((size_t)nn * B, 0.0f);
+  for (int i = 0; i < nn; ++i) {
+    const int o = order[i];
+    LightNode& d = nodes[i];
+    if (o < 0) {
+      std::memset(&d, 0, sizeof(d));
+      d.left = ~0;
+      d.right = -1;
+      d.parent = -1;
+      d.face = -1;
+      continue;
+    }
+    d = bl.nodes[o];
+    if (d.right >= 0) {
+      d.left = at[d.left];
+      d.right = at[d.right];
+    }
+    d.parent = d.parent >= 0 ? at[d.parent] : -1;
+    for (int b = 0; b < B; ++b) power[(size_t)b * nn + i] = (float)bl.npow[(size_t)o * B + b];
+  }
+  for (int i = 0; i < nn; ++i)
+    if (order[i] >= 0 && nodes[i].right < 0) keys.push_back(LightKey{nodes[i].face, i});
   std::sort(keys.begin(), keys.end(), [](const LightKey& a, const LightKey& b) { return a.face < b.face; });
-  return (int)nodes.size();
+  return nn;
 }

@@ -113,6 +113,7 @@ static void fill_scene_dev(const sr_handle* h, SceneDev& s) {
   s.lpower = h->n_lnodes > 0 ? (const float*)h->lpower.p : nullptr;
   s.lkeys = h->n_lkeys > 0 ? (const LightKey*)h->lkeys.p : nullptr;
   s.n_lkeys = h->n_lkeys;
+  s.n_lnodes = h->n_lnodes;
   s.B = h->B;
   s.max_lights = std::max(h->max_lights, 1);
   s.M = h->n_mats;

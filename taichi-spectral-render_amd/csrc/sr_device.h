@@ -113,10 +113,10 @@ struct SceneDev {
   const float* light_area; // [B]
   const float* lam_m;      // [B] band wavelength in metres (f32 as the reference forms it)
   const float* lam_nm;     // [B] WAVELENGTH_MIN + idx * STEP
-  const LightNode* lnodes; // light tree, preorder (root 0); many-light NEE only
-  const float* lpower;     // [lnodes][B] emitting area per band
+  const LightNode* lnodes; // light tree in sibling pairs (root 0, node 1 unused); many-light NEE only
+  const float* lpower;     // [B][n_lnodes] emitting area per band
   const LightKey* lkeys;   // [n_lkeys] emissive face -> leaf, sorted by face
-  int n_lkeys;
+  int n_lkeys, n_lnodes;
   int B, max_lights, M, n_tris;
   float cam_o[3], cam_u[3], cam_v[3], cam_w[3];
 };

@@ -457,10 +457,8 @@ __device__ __forceinline__ void finish_segment(Lane& L, const Ctx<ADJ>& c, float
 // and the MIS pdfs use A_light / p where the reference uses the band's
 // total emitting area (main.py:1351, 1355, 1556), which is the same thing
 // for the area-CDF pick.
-__device__ __forceinline__ float lt_importance(const SceneDev& s, int node, int band, f3 x) {
-  const float P = s.lpower[(size_t)node * s.B + band];
+__device__ __forceinline__ float lt_importance(const LightNode& nd, float P, f3 x) {
   if (!(P > 0.0f)) return 0.0f;
-  const LightNode& nd = s.lnodes[node];
   const f3 lo = mk(nd.lo[0], nd.lo[1], nd.lo[2]), hi = mk(nd.hi[0], nd.hi[1], nd.hi[2]);
   const f3 dv = x - (lo + hi) * 0.5f;
   const f3 ext = hi - lo;
@@ -468,13 +466,13 @@ __device__ __forceinline__ float lt_importance(const SceneDev& s, int node, int 
   const float d2 = dot(dv, dv);
   float cb = 1.0f;  // inside the bounding sphere: no angular bound
   if (d2 > r2) {
-    const float d = sqrtf(d2);
-    const float ct = fminf(1.0f, fmaxf(-1.0f, dot(mk(nd.axis[0], nd.axis[1], nd.axis[2]), dv) / d));
+    const float id = __builtin_amdgcn_rsqf(d2);
+    const float ct = fminf(1.0f, fmaxf(-1.0f, dot(mk(nd.axis[0], nd.axis[1], nd.axis[2]), dv) * id));
     if (ct < nd.cos_o) {  // x outside the normal cone: theta - theta_o in (0, pi]
-      const float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+      const float st = fsqrt(fmaxf(0.0f, 1.0f - ct * ct));
       const float cd = ct * nd.cos_o + st * nd.sin_o;  // cos(theta - theta_o)
       const float sd = st * nd.cos_o - ct * nd.sin_o;  // sin(theta - theta_o)
-      const float su = sqrtf(r2 / d2), cu = sqrtf(fmaxf(0.0f, 1.0f - r2 / d2));
+      const float su = fminf(1.0f, fsqrt(r2) * id), cu = fsqrt(fmaxf(0.0f, 1.0f - su * su));
       if (cd < cu) {  // beyond the sphere's angle too
         cb = cd * cu + sd * su;  // cos(theta - theta_o - theta_u)
         if (cb <= 0.0f) return 0.0f;
@@ -482,47 +480,66 @@ __device__ __forceinline__ float lt_importance(const SceneDev& s, int node, int 
     }
   }
   // distance to the centre, floored for points near the centre of a large node
-  return P * cb / fmaxf(d2, 1e-3f * r2 + 1e-12f);
+  return P * cb * frcp(fmaxf(d2, 1e-3f * r2 + 1e-12f));
+}
+
+// The nodes are stored in sibling pairs: the children of an inner node sit
+// at an even index f and f + 1 (root 0, node 1 unused), and the band's
+// powers of a pair are adjacent too, so each level of a descent or of the
+// pdf walk is one 128-byte node line plus one 8-byte power pair, both
+// addressed by the pair index alone.
+__device__ __forceinline__ float2 lt_power_pair(const SceneDev& s, int band, int f) {
+  return *reinterpret_cast<const float2*>(s.lpower + (size_t)band * s.n_lnodes + f);
 }
 
 // Descends from the root with one uniform; returns the leaf (or -1 if no
 // emitter below the root can face x) and its probability.
 __device__ int lt_sample(const SceneDev& s, int band, f3 x, float u, float& p) {
-  int node = 0;
   p = 1.0f;
-  while (s.lnodes[node].right >= 0) {
-    const int l = s.lnodes[node].left, r = s.lnodes[node].right;
-    const float il = lt_importance(s, l, band, x), ir = lt_importance(s, r, band, x);
+  if (s.n_lkeys == 1) return 0;  // the root is the only leaf
+  int f = 2;                     // the root's children
+  for (;;) {
+    const LightNode nl = s.lnodes[f], nr = s.lnodes[f + 1];
+    const float2 P = lt_power_pair(s, band, f);
+    const float il = lt_importance(nl, P.x, x), ir = lt_importance(nr, P.y, x);
     const float sum = il + ir;
     if (!(sum > 0.0f)) return -1;
-    const float pl = il / sum;
-    if (u < pl) {
-      u = fminf(u / pl, 0x1.fffffep-1f);
+    const float rs = frcp(sum);
+    const float pl = il * rs;
+    int node, next, right;
+    if (u < pl || !(ir > 0.0f)) {  // (il * rcp(il) may round below 1)
+      u = fminf(fdiv(u, pl), 0x1.fffffep-1f);
       p *= pl;
-      node = l;
+      node = f;
+      next = nl.left;
+      right = nl.right;
     } else {
-      const float pr = ir / sum;
-      u = fminf((u - pl) / pr, 0x1.fffffep-1f);
+      const float pr = ir * rs;
+      u = fminf(fdiv(u - pl, pr), 0x1.fffffep-1f);
       p *= pr;
-      node = r;
+      node = f + 1;
+      next = nr.left;
+      right = nr.right;
     }
+    if (right < 0) return node;
+    f = next;
   }
-  return node;
 }
 
-// Probability that lt_sample picks `leaf` from x.
+// Probability that lt_sample picks `leaf` from x: the walk up the parents,
+// each step reading the pair that holds the current node.
 __device__ float lt_pdf(const SceneDev& s, int band, f3 x, int leaf) {
   float p = 1.0f;
   int c = leaf;
-  int par = s.lnodes[c].parent;
-  while (par >= 0) {
-    const int l = s.lnodes[par].left, r = s.lnodes[par].right;
-    const float il = lt_importance(s, l, band, x), ir = lt_importance(s, r, band, x);
+  while (c > 0) {
+    const int f = c & ~1;
+    const LightNode nl = s.lnodes[f], nr = s.lnodes[f + 1];
+    const float2 P = lt_power_pair(s, band, f);
+    const float il = lt_importance(nl, P.x, x), ir = lt_importance(nr, P.y, x);
     const float sum = il + ir;
     if (!(sum > 0.0f)) return 0.0f;
-    p *= (c == l ? il : ir) / sum;
-    c = par;
-    par = s.lnodes[par].parent;
+    p *= (c == f ? il : ir) * frcp(sum);  // the pick probability exactly as lt_sample forms it
+    c = c == f ? nl.parent : nr.parent;
   }
   return p;
 }
@@ -788,7 +805,7 @@ __device__ __forceinline__ void on_extension(Lane& L, const Ctx<ADJ>& c, const H
         // the light tree's pick probability at the previous NEE vertex; a
         // face the band's NEE cannot pick keeps the reference's weight
         const int leaf = lt_leaf_of(s, h.prim);
-        if (leaf >= 0 && s.lpower[(size_t)leaf * s.B + L.active] > 0.0f) {
+        if (leaf >= 0 && s.lpower[(size_t)L.active * s.n_lnodes + leaf] > 0.0f) {
           const float p = lt_pdf(s, L.active, L.sdir, leaf);
           if (p > 0.0f) lden = cl * (s.shade[(size_t)h.prim * 4].w / p) + 1e-6f;
         }

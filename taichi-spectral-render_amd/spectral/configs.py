@@ -443,20 +443,25 @@ def closed_box(lo, hi, material_id, outward=True):
 
 
 def lamps(bands=25, wl_min=400.0, wl_max=700.0, width=32, height=32, spp=16, depth=4, seed=1, n=8, h=0.3,
-          size=0.1):
+          size=0.1, extent=3.5, floor_roughness=None):
     """Many-light scene: n x n small downward-facing lamps (emission 10) at
-    height h over a 4 x 4 diffuse floor whose index matches the air
-    (Fresnel 0), camera looking down at it."""
+    height h, spread evenly over an extent x extent square, above a diffuse
+    floor 0.5 wider whose index matches the air (Fresnel 0), camera looking
+    down at it.  With a large extent most lamps are far from any shading
+    point (the sparse many-light case).  floor_roughness (default: the
+    walls' roughness) 0 makes the floor Lambertian."""
     wl = band_wavelengths(wl_min, wl_max, bands)
     mats = [_mat(bands, n=1.0, tv=1),
-            _mat(bands, n=1.0, k=albedo_to_k(np.full(bands, 0.7), wl), r=WALL_ROUGHNESS, tv=0, ss=1.0),
+            _mat(bands, n=1.0, k=albedo_to_k(np.full(bands, 0.7), wl),
+                 r=WALL_ROUGHNESS if floor_roughness is None else floor_roughness, tv=0, ss=1.0),
             _mat(bands, n=1.0, se=1, em=10.0)]
-    meshes = [{"vertices": [[-2, 0, -2], [-2, 0, 2], [2, 0, 2], [2, 0, -2]], "faces": [[0, 1, 2], [0, 2, 3]],
+    f = 0.5 * extent + 0.25
+    meshes = [{"vertices": [[-f, 0, -f], [-f, 0, f], [f, 0, f], [f, 0, -f]], "faces": [[0, 1, 2], [0, 2, 3]],
                "material_id": 1}]
     for i in range(n):
         for j in range(n):
-            x0 = -1.75 + 3.5 * (i + 0.5) / n - size / 2
-            z0 = -1.75 + 3.5 * (j + 0.5) / n - size / 2
+            x0 = -0.5 * extent + extent * (i + 0.5) / n - size / 2
+            z0 = -0.5 * extent + extent * (j + 0.5) / n - size / 2
             meshes.append({"vertices": [[x0, h, z0], [x0 + size, h, z0], [x0 + size, h, z0 + size],
                                         [x0, h, z0 + size]], "faces": [[0, 1, 2], [0, 2, 3]], "material_id": 2})
     scene = {"materials": mats, "meshes": meshes,
