@@ -153,19 +153,27 @@ def test_sparse_lights_tree_same_expectation_far_lower_variance(gpu):
     assert ratio < 0.5
 
 
-def test_tree_matched_seed_parity_with_oracle(gpu):
+@pytest.mark.parametrize("scene", ["sparse_lamps", "emitting_sphere"])
+def test_tree_matched_seed_parity_with_oracle(gpu, scene):
     """The oracle's light-tree NEE (its own restatement of the descent and of
     the pdf walk, from the angles themselves rather than the kernel's
     angle-difference identities) fed the device's tree arrays: at matched
     seeds the GPU tree mode renders the oracle's image: 3 standard errors on
     99.7% of the pixel-bands and a mean relative error below 1e-4 (the two
     follow the same paths; they part only where f32 rounding of the
-    importance flips a pick), on the sparse lamps where the tree's picks and
-    MIS weights differ most from the CDF's."""
-    cfg = configs.lamps(n=16, extent=16.0)
-    flat = flatten_scene(cfg.scene, cfg.bands, cfg.wl_min, cfg.wl_max)
+    importance flips a pick).  Scenes: the sparse lamps, where the tree's
+    picks and MIS weights differ most from the CDF's, and the box with its
+    icosphere emitting in bands >= 8 only (per-band powers with empty
+    subtrees, cones of every width, glossy walls and a dielectric)."""
+    if scene == "sparse_lamps":
+        cfg = configs.lamps(n=16, extent=16.0)
+        flat = flatten_scene(cfg.scene, cfg.bands, cfg.wl_min, cfg.wl_max)
+        depth = 4
+    else:
+        cfg, flat = emitting_sphere_scene()
+        depth = cfg.depth
     W = H = 16
-    n_chunks, per, depth = 8, 64, 4
+    n_chunks, per = 8, 64
     r = Renderer(flat, cfg.bands, cfg.wl_min, cfg.wl_max, width=W, height=H, device=gpu)
     ora = OracleScene(flat)
     ora.light_tree(r.scene_array("light_tree"), r.scene_array("light_power").view(np.float32),
