@@ -122,7 +122,7 @@ int sr_scene_info(sr_handle* h, int* n_nodes, int* bvh_depth, int* max_lights, l
 #define SR_ARRAY_LVERTS 4      /* 3 x float4 per light slot */
 #define SR_ARRAY_LIGHT_COUNT 5 /* int[bands] */
 #define SR_ARRAY_LIGHT_AREA 6  /* float[bands] */
-#define SR_ARRAY_LIGHT_TREE 7  /* LightNode[light tree nodes] (64 B each), sibling pairs */
+#define SR_ARRAY_LIGHT_TREE 7  /* LightNode[light tree nodes] (64 B each: sphere, cone, links), sibling pairs */
 #define SR_ARRAY_LIGHT_POWER 8 /* float[bands][light tree nodes] */
 #define SR_ARRAY_LIGHT_KEYS 9  /* {face, leaf}[emissive faces], sorted by face */
 int sr_scene_array(sr_handle* h, int which, void* dst, size_t capacity, size_t* bytes);

@@ -154,13 +154,13 @@ struct Tri {
   float area;
 };
 struct LNode {  // one light-tree node, 64 bytes as the device stores it
-  float lo[3], cos_o, hi[3];
-  int left;  // inner: first child (the second is left + 1); leaf: ~slot
-  float axis[3];
+  float c[3], r2;  // bounding sphere: centre, squared radius
+  float axis[3], cos_o, sin_o;  // normal cone
+  float r;         // radius
+  int left;   // inner: first child (the second is left + 1); leaf: ~slot
   int right;  // inner: left + 1; leaf: -1
   int parent, face;
-  float sin_o;
-  int pad;
+  int pad[2];
 };
 static_assert(sizeof(LNode) == 64, "LNode mirrors the 64-byte device node");
 struct Mat {  // MaterialSample, main.py:277-289
@@ -897,17 +897,15 @@ static float shadow_transmittance(const Scene& s, V3 origin, V3 dir, float max_d
 // in proportion to their importance, with one uniform rescaled per level.
 static float lt_importance(const LNode& nd, float P, V3 x) {
   if (!(P > 0.0f)) return 0.0f;
-  V3 lo = v3(nd.lo[0], nd.lo[1], nd.lo[2]), hi = v3(nd.hi[0], nd.hi[1], nd.hi[2]);
-  V3 dv = x - (lo + hi) * 0.5f;
-  V3 ext = hi - lo;
-  float r2 = 0.25f * dot(ext, ext);
+  V3 dv = x - v3(nd.c[0], nd.c[1], nd.c[2]);
+  float r2 = nd.r2;
   float d2 = dot(dv, dv);
   float cb = 1.0f;
   if (d2 > r2) {
     float d = std::sqrt(d2);
     float th = std::acos(clampf(dot(v3(nd.axis[0], nd.axis[1], nd.axis[2]), dv) / d, -1.0f, 1.0f));
     float th_o = std::atan2(nd.sin_o, nd.cos_o);
-    float th_u = std::asin(tmin(1.0f, std::sqrt(r2) / d));
+    float th_u = std::asin(tmin(1.0f, nd.r / d));
     float a = tmax(0.0f, th - th_o - th_u);
     if (a >= 0.5f * PI_F) return 0.0f;
     cb = std::cos(a);

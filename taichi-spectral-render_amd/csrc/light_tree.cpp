@@ -6,7 +6,8 @@
 // splits of the slot centroids along the longest axis, each node holding the
 // box and the normal cone of the emitters below (Conty Estevez & Kulla,
 // "Importance Sampling of Many Lights with Adaptive Tree Splitting", 2018:
-// cone union) and the emitting area per band.  The kernel descends it with
+// cone union) and the emitting area per band; a node stores the sphere
+// around its box, which is all the importance uses.  The kernel descends it with
 // one uniform, choosing children in proportion to a conservative importance
 // (area x orientation bound / distance^2), so every emitter that can face
 // the shading point keeps a non-zero probability.
@@ -70,18 +71,21 @@ struct Builder {
   std::vector<LightNode> nodes;
   std::vector<double> npow;  // [nodes][B]
   std::vector<Cone> ncone;
+  std::vector<float> nbox;  // [nodes][6]: lo, hi
 
-  int build(int lo, int hi, int parent) {
+  int build(int lo_i, int hi_i, int parent) {
     const int me = (int)nodes.size();
     nodes.emplace_back();
     npow.resize(npow.size() + B, 0.0);
     ncone.emplace_back();
+    nbox.resize(nbox.size() + 6);
+    float lo[3], hi[3];
     LightNode nd;
     std::memset(&nd, 0, sizeof(nd));
     nd.parent = parent;
-    if (hi - lo == 1) {
-      const int s = idx[lo];
-      for (int k = 0; k < 3; ++k) { nd.lo[k] = box[s * 6 + k]; nd.hi[k] = box[s * 6 + 3 + k]; }
+    if (hi_i - lo_i == 1) {
+      const int s = idx[lo_i];
+      for (int k = 0; k < 3; ++k) { lo[k] = box[s * 6 + k]; hi[k] = box[s * 6 + 3 + k]; }
       nd.left = ~s;
       nd.right = -1;
       nd.face = face[s];
@@ -89,7 +93,7 @@ struct Builder {
       ncone[me] = cone[s];
     } else {
       double cl[3] = {1e300, 1e300, 1e300}, ch[3] = {-1e300, -1e300, -1e300};
-      for (int i = lo; i < hi; ++i)
+      for (int i = lo_i; i < hi_i; ++i)
         for (int k = 0; k < 3; ++k) {
           cl[k] = std::min(cl[k], cen[(size_t)idx[i] * 3 + k]);
           ch[k] = std::max(ch[k], cen[(size_t)idx[i] * 3 + k]);
@@ -97,14 +101,14 @@ struct Builder {
       int ax = 0;
       for (int k = 1; k < 3; ++k)
         if (ch[k] - cl[k] > ch[ax] - cl[ax]) ax = k;
-      std::stable_sort(idx.begin() + lo, idx.begin() + hi,
+      std::stable_sort(idx.begin() + lo_i, idx.begin() + hi_i,
                        [&](int a, int b) { return cen[(size_t)a * 3 + ax] < cen[(size_t)b * 3 + ax]; });
-      const int mid = lo + (hi - lo) / 2;
-      const int l = build(lo, mid, me);
-      const int r = build(mid, hi, me);
+      const int mid = lo_i + (hi_i - lo_i) / 2;
+      const int l = build(lo_i, mid, me);
+      const int r = build(mid, hi_i, me);
       for (int k = 0; k < 3; ++k) {
-        nd.lo[k] = std::min(nodes[l].lo[k], nodes[r].lo[k]);
-        nd.hi[k] = std::max(nodes[l].hi[k], nodes[r].hi[k]);
+        lo[k] = std::min(nbox[(size_t)l * 6 + k], nbox[(size_t)r * 6 + k]);
+        hi[k] = std::max(nbox[(size_t)l * 6 + 3 + k], nbox[(size_t)r * 6 + 3 + k]);
       }
       nd.left = l;
       nd.right = r;
@@ -112,6 +116,19 @@ struct Builder {
       for (int b = 0; b < B; ++b) npow[(size_t)me * B + b] = npow[(size_t)l * B + b] + npow[(size_t)r * B + b];
       ncone[me] = cone_union(ncone[l], ncone[r]);
     }
+    for (int k = 0; k < 3; ++k) {
+      nbox[(size_t)me * 6 + k] = lo[k];
+      nbox[(size_t)me * 6 + 3 + k] = hi[k];
+    }
+    // the sphere around the box, in the f32 operations (uncontracted) a
+    // kernel would use to form it
+    float ext[3];
+    for (int k = 0; k < 3; ++k) {
+      nd.c[k] = (lo[k] + hi[k]) * 0.5f;
+      ext[k] = hi[k] - lo[k];
+    }
+    nd.r2 = 0.25f * (ext[0] * ext[0] + ext[1] * ext[1] + ext[2] * ext[2]);
+    nd.r = std::sqrt(nd.r2);
     const Cone& c = ncone[me];
     for (int k = 0; k < 3; ++k) nd.axis[k] = (float)c.ax[k];
     // the half angle widened by kMargin (f32 rounding of the stored axis,
@@ -171,7 +188,7 @@ int sr_build_light_tree(int n, const float* lverts, const int* slots, const int*
     for (int b = 0; b < B; ++b)
       if (se[(size_t)slot_mat[s] * B + b]) pw[(size_t)s * B + b] = area;
   }
-  Builder bl{B, cen, box, cone, pw, face, {}, {}, {}, {}};
+  Builder bl{B, cen, box, cone, pw, face, {}, {}, {}, {}, {}};
   bl.idx.resize(n);
   for (int s = 0; s < n; ++s) bl.idx[s] = s;
   bl.nodes.reserve((size_t)2 * n);

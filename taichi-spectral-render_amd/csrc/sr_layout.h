@@ -46,22 +46,26 @@ struct alignas(16) LightRec {
   int pad;
 };
 
-// Light tree node (many-light NEE, SR_RENDER_LIGHT_TREE): bounding box and
-// normal cone of the emitters below -- axis and the cosine and sine of its
-// half angle widened by a safety margin (cos_o = -1: every direction);
-// inner: left/right children; leaf: left = ~slot, right = -1, face = its
-// original face id.  Per-band emitting area in a separate [nodes][B] array.
+// Light tree node (many-light NEE, SR_RENDER_LIGHT_TREE): bounding sphere
+// of the emitters below (the sphere around their bounding box: centre and
+// squared half diagonal, both in f32 as the kernel would form them, and the
+// radius) and their normal cone -- axis and the cosine and sine of its half
+// angle widened by a safety margin (cos_o = -1: every direction).  Inner:
+// the children at left and left + 1 = right; leaf: left = ~slot,
+// right = -1, face = its original face id.  The first 48 bytes are what the
+// descent reads.  Per-band emitting area in a separate [B][nodes] array.
 struct alignas(16) LightNode {
-  float lo[3];
-  float cos_o;
-  float hi[3];
-  int left;
+  float c[3];
+  float r2;
   float axis[3];
+  float cos_o;
+  float sin_o;
+  float r;
+  int left;
   int right;
   int parent;  // -1 at the root
   int face;    // leaf: original face id; inner: -1
-  float sin_o;
-  int pad;
+  int pad[2];
 };
 static_assert(sizeof(LightNode) == 64, "LightNode is 64 bytes");
 

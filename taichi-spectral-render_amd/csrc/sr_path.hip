@@ -459,10 +459,8 @@ __device__ __forceinline__ void finish_segment(Lane& L, const Ctx<ADJ>& c, float
 // for the area-CDF pick.
 __device__ __forceinline__ float lt_importance(const LightNode& nd, float P, f3 x) {
   if (!(P > 0.0f)) return 0.0f;
-  const f3 lo = mk(nd.lo[0], nd.lo[1], nd.lo[2]), hi = mk(nd.hi[0], nd.hi[1], nd.hi[2]);
-  const f3 dv = x - (lo + hi) * 0.5f;
-  const f3 ext = hi - lo;
-  const float r2 = 0.25f * dot(ext, ext);
+  const f3 dv = x - mk(nd.c[0], nd.c[1], nd.c[2]);
+  const float r2 = nd.r2;
   const float d2 = dot(dv, dv);
   float cb = 1.0f;  // inside the bounding sphere: no angular bound
   if (d2 > r2) {
@@ -472,7 +470,7 @@ __device__ __forceinline__ float lt_importance(const LightNode& nd, float P, f3 
       const float st = fsqrt(fmaxf(0.0f, 1.0f - ct * ct));
       const float cd = ct * nd.cos_o + st * nd.sin_o;  // cos(theta - theta_o)
       const float sd = st * nd.cos_o - ct * nd.sin_o;  // sin(theta - theta_o)
-      const float su = fminf(1.0f, fsqrt(r2) * id), cu = fsqrt(fmaxf(0.0f, 1.0f - su * su));
+      const float su = fminf(1.0f, nd.r * id), cu = fsqrt(fmaxf(0.0f, 1.0f - su * su));
       if (cd < cu) {  // beyond the sphere's angle too
         cb = cd * cu + sd * su;  // cos(theta - theta_o - theta_u)
         if (cb <= 0.0f) return 0.0f;

@@ -196,8 +196,9 @@ def test_tree_matched_seed_parity_with_oracle(gpu, scene):
     assert mre < 1e-4
 
 
-LIGHT_NODE = np.dtype([("lo", "<f4", 3), ("cos_o", "<f4"), ("hi", "<f4", 3), ("left", "<i4"), ("axis", "<f4", 3),
-                       ("right", "<i4"), ("parent", "<i4"), ("face", "<i4"), ("sin_o", "<f4"), ("pad", "<i4")])
+LIGHT_NODE = np.dtype([("c", "<f4", 3), ("r2", "<f4"), ("axis", "<f4", 3), ("cos_o", "<f4"), ("sin_o", "<f4"),
+                       ("r", "<f4"), ("left", "<i4"), ("right", "<i4"), ("parent", "<i4"), ("face", "<i4"),
+                       ("pad", "<i4", 2)])
 
 
 def emitting_sphere_scene():
@@ -219,8 +220,9 @@ def test_tree_layout_and_bounds(gpu):
     node 1 unused, the children of an inner node at an even index and the
     next, parents pointing back), every emitter reached once, per-band
     powers that add up the tree and total the light CDF's emitting area,
-    and the bounds the importance relies on: each node's box holds the
-    triangles below it and its normal cone the face normals below it."""
+    and the bounds the importance relies on: each node's sphere is the one
+    around the bounding box of the triangles below it (so it holds them)
+    and its normal cone holds the face normals below it."""
     cfg, flat = emitting_sphere_scene()
     B = cfg.bands
     r = Renderer(flat, B, cfg.wl_min, cfg.wl_max, width=8, height=8, device=gpu)
@@ -262,10 +264,23 @@ def test_tree_layout_and_bounds(gpu):
         leaves_below[i] = leaves_below[l] + leaves_below[rr]
         assert np.allclose(power[:, i], power[:, l] + power[:, rr], rtol=1e-6, atol=0)
     assert np.allclose(power[:, 0], area_cdf, rtol=1e-5)
+    lv32 = r.scene_array("lverts").view(np.float32).reshape(-1, 3, 4)[:, :, :3]
     for i in list(np.flatnonzero(inner)) + list(keys[:, 1]):
         s_ = np.array(leaves_below[i])
-        v = lv[s_].reshape(-1, 3)
-        assert (v >= nd["lo"][i]).all() and (v <= nd["hi"][i]).all(), i
+        # the sphere around the f32 bounding box of the triangles below,
+        # formed in f32 exactly as stored ...
+        v32 = lv32[s_].reshape(-1, 3)
+        lo, hi = v32.min(0), v32.max(0)
+        ext = hi - lo
+        assert np.array_equal(nd["c"][i], (lo + hi) * np.float32(0.5)), i
+        assert nd["r2"][i] == np.float32(0.25) * (ext[0] * ext[0] + ext[1] * ext[1] + ext[2] * ext[2]), i
+        assert nd["r"][i] == pytest.approx(np.sqrt(nd["r2"][i]), rel=1e-6)
+        # ... holds every vertex below, and the cone every face normal below
+        # (a vertex at a box corner lies on the sphere: allow the f32
+        # rounding of the centre, half an ulp per coordinate, and of r2)
+        d = np.sqrt(((lv[s_].reshape(-1, 3) - nd["c"][i].astype(np.float64)) ** 2).sum(1))
+        slack = np.sqrt(3.0) * 0.5 * np.spacing(np.abs(nd["c"][i])).max() + 1e-6 * float(nd["r"][i])
+        assert (d <= np.sqrt(float(nd["r2"][i])) + slack).all(), (i, (d - np.sqrt(float(nd["r2"][i]))).max(), slack)
         c = nrm[s_] @ nd["axis"][i].astype(np.float64)
         assert (c >= nd["cos_o"][i] - 1e-6).all(), (i, c.min(), nd["cos_o"][i])
     # the cones are not trivially wide: the root spans every direction, the

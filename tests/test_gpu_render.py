@@ -177,6 +177,45 @@ def test_upload_rejects_bad_material_ids(gpu):
         Renderer(flat, cfg.bands, cfg.wl_min, cfg.wl_max, device=gpu)
 
 
+def test_render_argument_limits(gpu):
+    """sr_render rejects what its kernels cannot index, before any launch:
+    more than 2^31 - 1 work items (W x H x n_blocks) in one call, negative
+    blocks or depth, stream ids beyond 24 bits (the RNG counter field) and
+    a zero resolution; a call without a scene is out of order.  Zero blocks
+    is a no-op that leaves the accumulator untouched."""
+    import ctypes
+
+    from spectral import _lib
+    from spectral._lib import SpectralRenderError, ptr
+    from spectral.render import Renderer
+
+    cfg = configs.box_icosphere()
+    flat = flatten_scene(cfg.scene, cfg.bands, cfg.wl_min, cfg.wl_max)
+    r = Renderer(flat, cfg.bands, cfg.wl_min, cfg.wl_max, width=8, height=8, device=gpu)
+    acc = ptr(r.accum)
+    bad = [
+        ((8, 8, 0, 0, 0, (1 << 31) // 64, 4, 0), "2\\^31"),
+        ((8, 8, 0, 0, -1, 1, 4, 0), "block range"),
+        ((8, 8, 0, 0, 0, -1, 4, 0), "block range"),
+        ((8, 8, 0, 0, 0, 1, -1, 0), "max_depth"),
+        ((8, 8, 0, 1 << 24, 0, 1, 4, 0), "24 bits"),
+        ((0, 8, 0, 0, 0, 1, 4, 0), "resolution"),
+    ]
+    for args, msg in bad:
+        with pytest.raises(SpectralRenderError, match=msg):
+            _lib.call("sr_render", r._h, *args, acc)
+    r.accum.fill_(7.0)
+    _lib.call("sr_render", r._h, 8, 8, 0, 0, 0, 0, 4, 0, acc)
+    assert bool((r.accum == 7.0).all())
+    h = ctypes.c_void_p()
+    _lib.call("sr_create", cfg.bands, float(cfg.wl_min), float(cfg.wl_max), ctypes.byref(h))
+    try:
+        with pytest.raises(SpectralRenderError, match="ESTATE"):
+            _lib.call("sr_render", h, 8, 8, 0, 0, 0, 1, 4, 0, acc)
+    finally:
+        _lib.call("sr_destroy", h)
+
+
 def test_empty_scene_renders_black(gpu):
     B = 4
     scene = {"materials": [configs._mat(B, n=1.0, tv=1)], "meshes": [],
