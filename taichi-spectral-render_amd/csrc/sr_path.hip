@@ -224,6 +224,14 @@ __device__ __forceinline__ void evaluate_bsdf_d(const SceneDev& s, const AdjMap&
   pdf = pdf + NdotL * 0.318309886184f * (1.0f - F) * opq;
 }
 
+// Free-flight distance of the lane's current walk step, kept in L.smax
+// (the shadow walk's field, unused until the walk's exit NEE); the
+// walk's steps are described at walk_step.
+__device__ __forceinline__ float walk_flight(const Job& a, const Lane& L, float sigma_t) {
+  const U4 u = draws(a, L, (L.seg << 9) | L.wstep, SR_SLOT_WALK);
+  return -logf(1.0f - u.u[0]) / sigma_t;
+}
+
 // scatter_surface, main.py:1000-1188, applied to the lane's ray as
 // cast_rays does (main.py:1565-1574).  `direct` is this segment's NEE
 // contribution (accumulated after the scatter), needed by the adjoint.
@@ -283,6 +291,7 @@ __device__ __forceinline__ void scatter_surface(Lane& L, const Ctx<ADJ>& c, floa
       L.walk = 1;
       L.wstep = 0;
       L.wmat = L.emat;
+      L.smax = walk_flight(a, L, sm.ss + absorb);
     } else if (!sm.tv) {
       dr = 1;
       branch = 3;
@@ -618,11 +627,17 @@ __device__ __forceinline__ void nee_setup(Lane& L, const Ctx<ADJ>& c, int surf, 
 
 // One step of a subsurface random walk inside material wmat (sigma_s, the
 // absorption 4 pi k / lambda and the HG g of its band, as the reference's
-// opaque branch forms them, main.py:1057-1058): a free flight; a boundary
-// hit before it ends is the exit point, which acts as a Lambertian vertex
-// (NEE, then a cosine lobe about the outward normal); otherwise an HG
-// scatter with weight sigma_s / sigma_t.  A walk that misses every surface
-// has left an open mesh and ends; kMaxWalk steps end it too.
+// opaque branch forms them, main.py:1057-1058): a free flight of length
+// t_k; a boundary hit within it is the exit point, which acts as a
+// Lambertian vertex (NEE, then a cosine lobe about the outward normal);
+// otherwise an HG scatter with weight sigma_s / sigma_t.  Step k's uniform
+// block (Philox slot SR_SLOT_WALK, counter seg << 9 | k) gives t_k and, for
+// k > 0, the direction leaving scatter k - 1, so t_k is known before step
+// k's trace: that trace only looks for hits up to t_k (t_init just above
+// it), which cuts most of the traversal deep inside the medium, and "no hit
+// reported" means "scatter at t_k".  A walk ends after kMaxWalk steps (in a
+// closed mesh it always meets the boundary first in practice; a walker that
+// has left an open mesh is ended by that limit).
 constexpr int kMaxWalk = 256;
 
 template <bool ADJ>
@@ -631,17 +646,7 @@ __device__ __forceinline__ void walk_step(Lane& L, const Ctx<ADJ>& c, const HitD
   const SceneDev& s = a.s;
   L.n_trav++;
   const int ab = L.active;
-  const MatRec wm = get_mat(s, L.wmat, ab);
-  const float sa = fdiv(4.0f * SR_PI * wm.k, s.lam_m[ab]);
-  const float stt = wm.ss + sa;
-  const int code = (L.seg << 9) | L.wstep;
-  const U4 u = draws(a, L, code, SR_SLOT_WALK);
-  const float t = -logf(1.0f - u.u[0]) / stt;
-  if (h.prim < 0) {
-    end_path<ADJ>(L, c);
-    return;
-  }
-  if (h.t <= t) {
+  if (h.prim >= 0) {  // the boundary within the flight: the exit point
     L.pos = L.pos + L.dir * h.t;
     const SurfHit sh = surface_of(s, h, L.dir);
     const f3 no = dot(sh.normal, L.dir) > 0.0f ? sh.normal : -sh.normal;
@@ -650,16 +655,21 @@ __device__ __forceinline__ void walk_step(Lane& L, const Ctx<ADJ>& c, const HitD
     L.efront = 1;
     L.eaction = 1;
     L.walk = 2;
-    nee_setup<ADJ>(L, c, -2, code, SR_SLOT_WALK_NEE);
+    nee_setup<ADJ>(L, c, -2, (L.seg << 9) | L.wstep, SR_SLOT_WALK_NEE);
     return;
   }
-  L.pos = L.pos + L.dir * t;
+  const MatRec wm = get_mat(s, L.wmat, ab);
+  const float sa = fdiv(4.0f * SR_PI * wm.k, s.lam_m[ab]);
+  const float stt = wm.ss + sa;
+  L.pos = L.pos + L.dir * L.smax;
   L.tp *= wm.ss / stt;
-  L.dir = hg_sample(L.dir, wm.g, u.u[1], u.u[2]);
   if (++L.wstep >= kMaxWalk) {
     end_path<ADJ>(L, c);
     return;
   }
+  const U4 u = draws(a, L, (L.seg << 9) | L.wstep, SR_SLOT_WALK);
+  L.dir = hg_sample(L.dir, wm.g, u.u[1], u.u[2]);
+  L.smax = -logf(1.0f - u.u[0]) / stt;
   L.phase = PH_EXT;
 }
 
@@ -1059,6 +1069,7 @@ __device__ __forceinline__ void path_loop(const PathArgs& P, int* stk, float* ld
       f3 ro = do_ext ? L.pos : L.pos + L.sdir * L.soff;
       f3 rd = do_ext ? L.dir : L.sdir;
       float t_init = 1e9f;
+      if (SSS && do_ext && L.walk == 1) t_init = nextafterf(L.smax, INFINITY);  // hits at t <= the flight
       if (!do_ext) {
         // A shadow step through a medium with sigma_a + sigma_s = 0 needs
         // only the hits before the light: any hit at t >= smax - strav ends
