@@ -224,12 +224,14 @@ __device__ __forceinline__ void evaluate_bsdf_d(const SceneDev& s, const AdjMap&
   pdf = pdf + NdotL * 0.318309886184f * (1.0f - F) * opq;
 }
 
-// Free-flight distance of the lane's current walk step, kept in L.smax
-// (the shadow walk's field, unused until the walk's exit NEE); the
-// walk's steps are described at walk_step.
-__device__ __forceinline__ float walk_flight(const Job& a, const Lane& L, float sigma_t) {
-  const U4 u = draws(a, L, (L.seg << 9) | L.wstep, SR_SLOT_WALK);
-  return -logf(1.0f - u.u[0]) / sigma_t;
+// A subsurface walk keeps its medium and its next flight in the shadow
+// walk's fields, which are unused until the walk's exit NEE: L.stot =
+// sigma_t, L.strav = sigma_s / sigma_t, L.soff = the HG g, L.smax = the
+// current step's flight length.  The walk's steps are described at
+// walk_step; it has no counterpart in the reference, so its arithmetic
+// uses the hardware logarithm and reciprocal.
+__device__ __forceinline__ float walk_flight(float u0, float sigma_t) {
+  return -__logf(1.0f - u0) * frcp(sigma_t);
 }
 
 // scatter_surface, main.py:1000-1188, applied to the lane's ray as
@@ -291,7 +293,10 @@ __device__ __forceinline__ void scatter_surface(Lane& L, const Ctx<ADJ>& c, floa
       L.walk = 1;
       L.wstep = 0;
       L.wmat = L.emat;
-      L.smax = walk_flight(a, L, sm.ss + absorb);
+      L.stot = sm.ss + absorb;
+      L.strav = fdiv(sm.ss, L.stot);
+      L.soff = sm.g;
+      L.smax = walk_flight(draws(a, L, L.seg << 9, SR_SLOT_WALK).u[0], L.stot);
     } else if (!sm.tv) {
       dr = 1;
       branch = 3;
@@ -645,7 +650,6 @@ __device__ __forceinline__ void walk_step(Lane& L, const Ctx<ADJ>& c, const HitD
   const Job& a = c.a;
   const SceneDev& s = a.s;
   L.n_trav++;
-  const int ab = L.active;
   if (h.prim >= 0) {  // the boundary within the flight: the exit point
     L.pos = L.pos + L.dir * h.t;
     const SurfHit sh = surface_of(s, h, L.dir);
@@ -658,18 +662,15 @@ __device__ __forceinline__ void walk_step(Lane& L, const Ctx<ADJ>& c, const HitD
     nee_setup<ADJ>(L, c, -2, (L.seg << 9) | L.wstep, SR_SLOT_WALK_NEE);
     return;
   }
-  const MatRec wm = get_mat(s, L.wmat, ab);
-  const float sa = fdiv(4.0f * SR_PI * wm.k, s.lam_m[ab]);
-  const float stt = wm.ss + sa;
   L.pos = L.pos + L.dir * L.smax;
-  L.tp *= wm.ss / stt;
+  L.tp *= L.strav;
   if (++L.wstep >= kMaxWalk) {
     end_path<ADJ>(L, c);
     return;
   }
   const U4 u = draws(a, L, (L.seg << 9) | L.wstep, SR_SLOT_WALK);
-  L.dir = hg_sample(L.dir, wm.g, u.u[1], u.u[2]);
-  L.smax = -logf(1.0f - u.u[0]) / stt;
+  L.dir = hg_sample(L.dir, L.soff, u.u[1], u.u[2]);
+  L.smax = walk_flight(u.u[0], L.stot);
   L.phase = PH_EXT;
 }
 
