@@ -4,13 +4,13 @@
 // leaves a light BVH as a TODO (main.py:541).  Here the light slots (the
 // emissive faces, main.py:590-627) are organised as a binary tree: median
 // splits of the slot centroids along the longest axis, each node holding the
-// box and the normal cone of the emitters below (Conty Estevez & Kulla,
-// "Importance Sampling of Many Lights with Adaptive Tree Splitting", 2018:
-// cone union) and the emitting area per band; a node stores the sphere
-// around its box, which is all the importance uses.  The kernel descends it with
-// one uniform, choosing children in proportion to a conservative importance
-// (area x orientation bound / distance^2), so every emitter that can face
-// the shading point keeps a non-zero probability.
+// bounding sphere (the one around the bounding box) and the normal cone of
+// the emitters below (Conty Estevez & Kulla, "Importance Sampling of Many
+// Lights with Adaptive Tree Splitting", 2018: cone union) and the emitting
+// area per band.  The kernel descends it with one uniform, choosing
+// children in proportion to a conservative importance (area x orientation
+// bound / distance^2), so every emitter that can face the shading point
+// keeps a non-zero probability.
 //
 // Deterministic (stable sort, fixed arithmetic), so host and device scene
 // builds, which both call it, produce the same arrays.
@@ -69,7 +69,7 @@ struct Builder {
   const std::vector<int>& face;      // [n]
   std::vector<int> idx;
   std::vector<LightNode> nodes;
-  std::vector<double> npow;  // [nodes][B]
+  std::vector<double> npow;  // [nodes][B] while building; stored [B][nodes]
   std::vector<Cone> ncone;
   std::vector<float> nbox;  // [nodes][6]: lo, hi
 
