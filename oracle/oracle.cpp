@@ -25,9 +25,13 @@
 //   light-tree NEE (optional)   the build's own extension (main.py:541 is a
 //                               TODO): the tree arrays are given, the
 //                               descent and pdf walk are restated from their
-//                               definition in IEEE f32; pinned by the
-//                               structural and expectation tests of the GPU
-//                               tree mode (tests/test_gpu_lighttree.py)
+//                               definition in IEEE f32; pinned on a tree
+//                               built in the test (probabilities sum to at
+//                               most 1, and to 1 where every light faces the
+//                               point; stratified sampler frequencies equal
+//                               the pdf; lights facing the point keep p > 0;
+//                               tests/test_oracle_lighttree.py) and by the
+//                               GPU tree mode's expectation tests
 //
 // Deliberate, documented deviations shared with the HIP path:
 //   * light CDF last entry forced to 1.0 (reference main.py:619-621 leaves
@@ -1265,6 +1269,29 @@ int or_render(void* p, int W, int H, uint64_t seed, uint32_t stream, int first_b
   }
   if (stats) { stats[0] = (double)seg; stats[1] = (double)trav; stats[2] = (double)nf; }
   return 0;
+}
+
+// Light-tree pick on given arrays (nodes as 64-byte records in sibling
+// pairs, power [B][n]): for each of the k points x[k][3] and uniforms u[k],
+// the leaf lt_sample picks (-1: none) and its probability; or_lt_pdf gives
+// lt_pdf of leaves[k] at x[k].
+static Scene lt_scene(int n, const void* nodes, const float* power, int B) {
+  Scene s;
+  s.B = B;
+  s.lt_nodes.resize(n);
+  std::memcpy(s.lt_nodes.data(), nodes, (size_t)n * sizeof(LNode));
+  s.lt_power.assign(power, power + (size_t)B * n);
+  return s;
+}
+void or_lt_sample(int n, const void* nodes, const float* power, int B, int band, int k, const float* x,
+                  const float* u, int* leaf, float* prob) {
+  const Scene s = lt_scene(n, nodes, power, B);
+  for (int i = 0; i < k; ++i) leaf[i] = lt_sample(s, band, v3(x[3 * i], x[3 * i + 1], x[3 * i + 2]), u[i], prob[i]);
+}
+void or_lt_pdf(int n, const void* nodes, const float* power, int B, int band, int k, const float* x, const int* leaves,
+               float* prob) {
+  const Scene s = lt_scene(n, nodes, power, B);
+  for (int i = 0; i < k; ++i) prob[i] = lt_pdf(s, band, v3(x[3 * i], x[3 * i + 1], x[3 * i + 2]), leaves[i]);
 }
 
 // Unit-level entry points for closed-form tests.

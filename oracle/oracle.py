@@ -46,6 +46,9 @@ def load():
         lib.or_shadow.argtypes = [_vp, _vp, _vp, ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         lib.or_nee.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
                                ctypes.c_int, _vp]
+        lib.or_lt_sample.argtypes = [ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp,
+                                     _vp]
+        lib.or_lt_pdf.argtypes = [ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]
         lib.or_volume_events.argtypes = [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, _vp,
                                          _vp, _vp, _vp]
         _LIB = lib
@@ -138,6 +141,33 @@ class OracleScene:
         load().or_volume_events(self.h, _p(_f32(pos)), _p(_f32(dirn)), int(vol), int(band), int(seed), int(n), _p(tr),
                                 _p(ac), _p(es), _p(nb))
         return tr, ac, es, nb
+
+
+def lt_sample(nodes, power, band, x, u):
+    """Light-tree picks for points x [k, 3] and uniforms u [k]: (leaf, prob)."""
+    nodes = np.ascontiguousarray(nodes, np.uint8)
+    power = _f32(power)
+    x = _f32(x).reshape(-1, 3)
+    u = _f32(u).reshape(-1)
+    k = x.shape[0]
+    leaf = np.empty(k, np.int32)
+    prob = np.empty(k, np.float32)
+    n = nodes.nbytes // 64
+    load().or_lt_sample(n, _p(nodes), _p(power), power.size // n, int(band), k, _p(x), _p(u), _p(leaf), _p(prob))
+    return leaf, prob
+
+
+def lt_pdf(nodes, power, band, x, leaves):
+    """Probability that the light-tree pick from x [k, 3] returns leaves [k]."""
+    nodes = np.ascontiguousarray(nodes, np.uint8)
+    power = _f32(power)
+    x = _f32(x).reshape(-1, 3)
+    lv = np.ascontiguousarray(leaves, np.int32).reshape(-1)
+    k = x.shape[0]
+    prob = np.empty(k, np.float32)
+    n = nodes.nbytes // 64
+    load().or_lt_pdf(n, _p(nodes), _p(power), power.size // n, int(band), k, _p(x), _p(lv), _p(prob))
+    return prob
 
 
 def philox(ctr, key):
