@@ -274,9 +274,11 @@ def bench_rows(reps=5, log=print):
     pixel; RGBA8 conversion: 12 B read + 4 B written per pixel; CMF resolve
     of the 31-band accumulator: 124 B read + 12 B written per pixel), NEE through
     the light tree against the area CDF (time, pixel variance over 8 blocks and
-    efficiency = 1 / (variance x time)) on 128 lamps over 3.5 units and on
-    1024 lamps over 32 units, and the subsurface walk on the closed slab
-    (w = 0.9), all at 1024x1024 so that one block fills the persistent grid."""
+    efficiency = 1 / (variance x time)) on 128 lamps over 3.5 units, on
+    1024 lamps over 32 units and on the box whose icosphere emits in bands
+    >= 8 (1282 lights facing every way), and the subsurface walk on the
+    closed slab (w = 0.9), all at 1024x1024 so that one block fills the
+    persistent grid."""
     import numpy as np
     import torch
 
@@ -318,8 +320,11 @@ def bench_rows(reps=5, log=print):
     out["resolve"] = {"bytes": rb, "ms": round(ms_r, 4), "GB_per_s": round(rb / ms_r / 1e6, 1),
                       "frac_of_8TBps": round(rb / ms_r / 1e6 / 8000, 3)}
     del r
+    sphere = configs.box_icosphere(width=1024, height=1024)
+    sphere.scene["materials"][5] = configs.emitting_sphere_material(sphere.bands, sphere.wl_min, sphere.wl_max)
     rows = (("light_tree", configs.lamps(width=1024, height=1024)),
             ("light_tree_sparse", configs.lamps(width=1024, height=1024, n=32, extent=32.0)),
+            ("light_tree_sphere", sphere),
             ("subsurface", configs.subsurface_slab(width=1024, height=1024)))
     for name, c in rows:
         f = flatten_scene(c.scene, c.bands, c.wl_min, c.wl_max)

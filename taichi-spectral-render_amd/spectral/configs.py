@@ -442,6 +442,16 @@ def closed_box(lo, hi, material_id, outward=True):
     return {"vertices": v, "faces": faces, "material_id": material_id}
 
 
+def emitting_sphere_material(bands, wl_min, wl_max):
+    """The box scene's icosphere (material 5) made an emitter in bands >= 8
+    only: a rough n = 1.4 opaque surface of albedo 0.5 with emission 3, so
+    its 1280 faces are lights in those bands and the light lists, CDFs and
+    light-tree powers differ by band."""
+    se = (np.arange(bands) >= 8).astype(int)
+    k = albedo_to_k(np.full(bands, 0.5), band_wavelengths(wl_min, wl_max, bands))
+    return _mat(bands, n=1.4, k=k, r=0.2, tv=0, ss=1.0, se=se, em=3.0)
+
+
 def lamps(bands=25, wl_min=400.0, wl_max=700.0, width=32, height=32, spp=16, depth=4, seed=1, n=8, h=0.3,
           size=0.1, extent=3.5, floor_roughness=None):
     """Many-light scene: n x n small downward-facing lamps (emission 10) at

@@ -205,14 +205,9 @@ def emitting_sphere_scene():
     """The box scene with its icosphere emitting in bands >= 8: 1282 light
     slots with normals in every direction and per-band emitting areas that
     differ between the ceiling light and the sphere."""
-    from spectral.data import band_wavelengths
-
     cfg = configs.box_icosphere()
-    B = cfg.bands
-    se = (np.arange(B) >= 8).astype(int)
-    k = configs.albedo_to_k(np.full(B, 0.5), band_wavelengths(cfg.wl_min, cfg.wl_max, B))
-    cfg.scene["materials"][5] = configs._mat(B, n=1.4, k=k, r=0.2, tv=0, ss=1.0, se=se, em=3.0)
-    return cfg, flatten_scene(cfg.scene, B, cfg.wl_min, cfg.wl_max)
+    cfg.scene["materials"][5] = configs.emitting_sphere_material(cfg.bands, cfg.wl_min, cfg.wl_max)
+    return cfg, flatten_scene(cfg.scene, cfg.bands, cfg.wl_min, cfg.wl_max)
 
 
 def test_tree_layout_and_bounds(gpu):

@@ -257,14 +257,9 @@ def test_many_lights_matched_seed_parity(gpu):
     and slots differ by band.  The GPU picks lights by binary search over
     the CDF, the oracle by the reference's linear scan (main.py:1310-1313);
     both pick the same light for the same uniform, so matched seeds agree."""
-    from spectral.data import band_wavelengths
-
     cfg = configs.box_icosphere()
     B = cfg.bands
-    se = (np.arange(B) >= 8).astype(int)
-    cfg.scene["materials"][5] = configs._mat(B, n=1.4, k=configs.albedo_to_k(np.full(B, 0.5),
-                                                                              band_wavelengths(cfg.wl_min, cfg.wl_max, B)),
-                                             r=0.2, tv=0, ss=1.0, se=se, em=3.0)
+    cfg.scene["materials"][5] = configs.emitting_sphere_material(B, cfg.wl_min, cfg.wl_max)
     flat = flatten_scene(cfg.scene, B, cfg.wl_min, cfg.wl_max)
     ora = OracleScene(flat)
     W = H = 8
