@@ -149,9 +149,10 @@ class Renderer:
         """Run blocks [blocks_done, blocks_done + n_blocks) and fold them into
         the running spectral mean ``accum`` [W, H, B] (cast_rays x B +
         average_and_reset_block per block, main.py:1826-1836).
-        ``light_tree``: NEE picks lights through the light tree (same
-        expected image, lower variance with many lights) instead of the
-        reference's area CDF.  ``subsurface``: opaque materials with a
+        ``light_tree``: NEE picks lights through the light tree instead of
+        the reference's area CDF (the same expected image wherever the
+        reference's NEE and BSDF strategies estimate the same integral; lower
+        variance with many small lights).  ``subsurface``: opaque materials with a
         scattering coefficient scatter by a random walk inside their (closed)
         mesh instead of the single-scatter Oren-Nayar lobe."""
         if self.accum is None:

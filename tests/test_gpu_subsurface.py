@@ -11,7 +11,9 @@ image to match; the walk is pinned by radiative-transfer closed forms:
   many mean free paths thick;
 * a closed sphere with w = 1 under the same sky returns exactly L (white
   furnace: every walk leaves, nothing is absorbed);
-* with no opaque scattering material the flag changes nothing.
+* with no opaque scattering material the flag changes nothing;
+* with the light tree as well (both sr_render flags), the image's energy is
+  the one of the walk with the area-CDF pick.
 The H-function solver itself is checked on the CPU against its identities
 and Chandrasekhar's tabulated H(1) = 2.9078 for w = 1."""
 import math
@@ -119,3 +121,24 @@ def test_flag_is_a_no_op_without_scattering_opaque_materials(gpu):
         r.render(n_blocks=8, max_depth=cfg.depth, seed=2, subsurface=sss)
         imgs.append(r.accum.cpu().numpy().astype(np.float64))
     assert np.allclose(imgs[0], imgs[1], rtol=1e-5, atol=1e-7)
+
+
+@pytest.mark.gpu
+def test_walk_with_light_tree_same_expectation(gpu):
+    # the sky box's 12 faces are the lights; the walk's exit NEE picks them
+    # through the tree in one mode and by the area CDF in the other
+    flat = _scene(_box([-10, -5, -10], [10, 0, 10], 1), 0.9, sigma_s=5.0)
+    r = Renderer(flat, B, 400.0, 700.0, width=16, height=16, device=gpu)
+    tot = {}
+    for tree in (False, True):
+        vals = []
+        for c in range(32):
+            r.accum.zero_()
+            r.blocks_done = c * 16
+            r.render(n_blocks=16, max_depth=5, seed=4, stream=1 if tree else 0, subsurface=True, light_tree=tree)
+            vals.append(float(r.accum.double().sum()) * (c + 1))
+        tot[tree] = np.array(vals)
+    se = math.sqrt(tot[False].var(ddof=1) / 32 + tot[True].var(ddof=1) / 32)
+    m0, m1 = tot[False].mean(), tot[True].mean()
+    print(f"tree / CDF with the walk: {m1 / m0:.5f} +- {se / m0:.5f}")
+    assert abs(m1 - m0) < 4 * se + 5e-4 * m0
