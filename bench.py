@@ -268,7 +268,7 @@ def bench_build(reps=3, log=print):
     return out
 
 
-def bench_rows(reps=5, log=print):
+def bench_rows(reps=5, log=print, only=None):
     """Throughput of the rows around the path: the display path on
     heightfield_1m at 1024x1024 (normals view: one closed-hit trace per
     pixel; RGBA8 conversion: 12 B read + 4 B written per pixel; CMF resolve
@@ -300,6 +300,9 @@ def bench_rows(reps=5, log=print):
         return best
 
     out = {"metric": "display path, light-tree NEE and subsurface walk throughput"}
+    if only is not None:
+        out["metric"] = f"{only} row of the display / light-tree / subsurface rows"
+        return bench_rows_paths(out, reps, only)
     cfg = configs.heightfield_1m()
     flat = flatten_scene(cfg.scene, cfg.bands, cfg.wl_min, cfg.wl_max)
     r = Renderer(flat, cfg.bands, cfg.wl_min, cfg.wl_max, width=1024, height=1024, device="cuda:0")
@@ -320,6 +323,18 @@ def bench_rows(reps=5, log=print):
     out["resolve"] = {"bytes": rb, "ms": round(ms_r, 4), "GB_per_s": round(rb / ms_r / 1e6, 1),
                       "frac_of_8TBps": round(rb / ms_r / 1e6 / 8000, 3)}
     del r
+    return bench_rows_paths(out, reps, None)
+
+
+def bench_rows_paths(out, reps, only):
+    """The light-tree and subsurface rows of bench_rows (all, or the one
+    named `only`)."""
+    import numpy as np
+
+    from spectral import configs
+    from spectral.render import Renderer
+    from spectral.scene import flatten_scene
+
     sphere = configs.box_icosphere(width=1024, height=1024)
     sphere.scene["materials"][5] = configs.emitting_sphere_material(sphere.bands, sphere.wl_min, sphere.wl_max)
     rows = (("light_tree", configs.lamps(width=1024, height=1024)),
@@ -327,6 +342,8 @@ def bench_rows(reps=5, log=print):
             ("light_tree_sphere", sphere),
             ("subsurface", configs.subsurface_slab(width=1024, height=1024)))
     for name, c in rows:
+        if only is not None and name != only:
+            continue
         f = flatten_scene(c.scene, c.bands, c.wl_min, c.wl_max)
         rr = Renderer(f, c.bands, c.wl_min, c.wl_max, width=1024, height=1024, device="cuda:0")
         light = name.startswith("light_tree")
@@ -371,7 +388,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=16)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="heightfield_1m")
+    ap.add_argument("--workload", default="heightfield_1m",
+                    help="heightfield_1m (the headline), trace, build, batch64, rows, or one path row: "
+                         "light_tree, light_tree_sparse, light_tree_sphere, subsurface")
     ap.add_argument("--res", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -398,6 +417,9 @@ def main():
         return
     if args.workload == "rows":
         print(json.dumps(bench_rows()), flush=True)
+        return
+    if args.workload in ("light_tree", "light_tree_sparse", "light_tree_sphere", "subsurface"):
+        print(json.dumps(bench_rows(only=args.workload)), flush=True)
         return
     if args.workload == "build":
         print(json.dumps(bench_build()), flush=True)
