@@ -22,10 +22,6 @@ from . import _lib
 from .scene import FlatScene, flatten_scene, pixel_center_rays, scene_dimensions
 
 
-def _stream_ptr():
-    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-
-
 class Renderer:
     """One scene, one accumulator.
 
@@ -74,7 +70,12 @@ class Renderer:
             pass
 
     def _bind_stream(self):
-        _lib.call("sr_set_stream", self._h, _stream_ptr())
+        """Point the handle at torch's current stream (one ctypes call only
+        when that stream changed since the last call)."""
+        st = torch.cuda.current_stream(self.device).cuda_stream
+        if st != getattr(self, "_bound_stream", None):
+            _lib.call("sr_set_stream", self._h, ctypes.c_void_p(st))
+            self._bound_stream = st
 
     def _upload(self, f: FlatScene):
         self._bind_stream()
