@@ -272,7 +272,8 @@ def bench_rows(reps=5, log=print, only=None):
     """Throughput of the rows around the path: the display path on
     heightfield_1m at 1024x1024 (normals view: one closed-hit trace per
     pixel; RGBA8 conversion: 12 B read + 4 B written per pixel; CMF resolve
-    of the 31-band accumulator: 124 B read + 12 B written per pixel), NEE through
+    of the 31-band accumulator: 124 B read + 12 B written per pixel, and from
+    a 512 x 512 accumulator to a 1024 x 1024 display), NEE through
     the light tree against the area CDF (time, pixel variance over 8 blocks and
     efficiency = 1 / (variance x time)) on 128 lamps over 3.5 units, on
     1024 lamps over 32 units and on the box whose icosphere emits in bands
@@ -322,6 +323,14 @@ def bench_rows(reps=5, log=print, only=None):
     rb = npx * (cfg.bands + 3) * 4
     out["resolve"] = {"bytes": rb, "ms": round(ms_r, 4), "GB_per_s": round(rb / ms_r / 1e6, 1),
                       "frac_of_8TBps": round(rb / ms_r / 1e6 / 8000, 3)}
+    # the reference's default display: the render grid shown at twice its
+    # size (RESOLUTION_SCALE = 2): a 512 x 512 accumulator to 1024 x 1024
+    r.set_resolution(512, 512)
+    r.accum.uniform_()
+    ms_u = timed(lambda: [r.color_rays(cs, out_width=1024, out_height=1024) for _ in range(20)]) / 20
+    ub = 512 * 512 * cfg.bands * 4 + npx * 12
+    out["resolve_x2"] = {"bytes": ub, "ms": round(ms_u, 4), "GB_per_s": round(ub / ms_u / 1e6, 1),
+                         "frac_of_8TBps": round(ub / ms_u / 1e6 / 8000, 3)}
     del r
     return bench_rows_paths(out, reps, None)
 

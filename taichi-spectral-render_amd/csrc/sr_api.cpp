@@ -87,6 +87,7 @@ struct sr_handle {
   int n_lnodes = 0, n_lkeys = 0;
   DevBuf nodes, tris, shade, mats, mat_eem, eem, lights, lverts, light_count, light_area, lam_m, lam_nm;
   DevBuf lnodes, lpower, lkeys;  // light tree
+  DevBuf rgb_tmp;                // resolved render-grid image for an upsampled display
   std::vector<float> h_light_area;
   std::vector<int> h_light_count;
   float cam_o[3], cam_u[3], cam_v[3], cam_w[3];
@@ -272,7 +273,7 @@ int sr_destroy(sr_handle* h) {
   if (h->st) (void)hipStreamSynchronize(h->st);
   DevBuf* bufs[] = {&h->nodes, &h->tris, &h->shade, &h->mats, &h->mat_eem, &h->eem, &h->lights, &h->lverts,
                     &h->light_count, &h->light_area, &h->lam_m, &h->lam_nm, &h->lnodes, &h->lpower, &h->lkeys,
-                    &h->sum, &h->work, &h->jobs, &h->overflow, &h->tmp, &h->alb_off};
+                    &h->sum, &h->work, &h->jobs, &h->overflow, &h->tmp, &h->alb_off, &h->rgb_tmp};
   for (DevBuf* b : bufs) b->release();
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -810,7 +811,13 @@ int sr_resolve(sr_handle* h, const float* dev_accum, int width, int height, cons
   p.norm = norm;
   p.exposure = exposure;
   p.gamma = gamma;
-  if (sr_launch_resolve(dev_accum, width, height, h->B, p, dev_rgb, out_w, out_h, h->st))
+  float* tmp = nullptr;
+  if (out_w != width || out_h != height) {
+    int rc = h->rgb_tmp.ensure((size_t)width * height * 3 * sizeof(float));
+    if (rc) return rc;
+    tmp = (float*)h->rgb_tmp.p;
+  }
+  if (sr_launch_resolve(dev_accum, width, height, h->B, p, dev_rgb, out_w, out_h, tmp, h->st))
     return fail(SR_EHIP, "resolve launch failed");
   return 0;
 }

@@ -100,8 +100,10 @@ int sr_launch_path_sss(const PathArgs& a, int grid, int block, hipStream_t st);
 int sr_launch_fill_u32(uint32_t* p, size_t n_words, uint32_t v, hipStream_t st);
 int sr_launch_finalize(float* mean, float* sum, size_t n, int first, int nb, hipStream_t st);
 int sr_launch_fill(float* p, size_t n, float v, hipStream_t st);
+// tmp: W * H * 3 floats of scratch for an upsampled display (OW, OH != W, H),
+// or null (then each display pixel resolves its render pixel itself).
 int sr_launch_resolve(const float* acc, int W, int H, int B, const ResolveParams& p, float* out, int OW, int OH,
-                      hipStream_t st);
+                      float* tmp, hipStream_t st);
 int sr_launch_normals(const SceneDev& s, int W, int H, int OW, int OH, float* out, uint32_t* overflow,
                       hipStream_t st);
 int sr_launch_rgba8(const float* rgb, size_t n, void* out, hipStream_t st);

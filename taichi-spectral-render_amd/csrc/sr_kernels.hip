@@ -95,6 +95,23 @@ __global__ __launch_bounds__(256) void sr_resolve_tile_kernel(const float* __res
   if ((int)threadIdx.x < np) resolve_pixel(tile + threadIdx.x * B, B, p, out + (size_t)(p0 + threadIdx.x) * 3);
 }
 
+// Nearest-neighbour display upsample of a resolved [W][H][3] image, with the
+// render pixel chosen exactly as sr_resolve_kernel chooses it
+// (main.py:1614-1615).
+__global__ void sr_upsample_rgb_kernel(const float* __restrict__ rgb, int W, int H, float* __restrict__ out, int OW,
+                                       int OH) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= OW * OH) return;
+  const int i = idx / OH, j = idx % OH;
+  const int ri = (int)((float)i / (float)OW * (float)W);
+  const int rj = (int)((float)j / (float)OH * (float)H);
+  const float* s = rgb + ((size_t)ri * H + rj) * 3;
+  float* d = out + (size_t)idx * 3;
+  d[0] = s[0];
+  d[1] = s[1];
+  d[2] = s[2];
+}
+
 // ====================================================================== display
 // visualize_normals, main.py:1632-1660: a pixel-centre camera ray per render
 // pixel, colour = shading normal * 0.5 + 0.5 (black on a miss), then the
@@ -181,12 +198,19 @@ int sr_launch_fill(float* p, size_t n, float v, hipStream_t st) {
 }
 
 int sr_launch_resolve(const float* acc, int W, int H, int B, const ResolveParams& p, float* out, int OW, int OH,
-                      hipStream_t st) {
+                      float* tmp, hipStream_t st) {
   int n = OW * OH;
   if (n <= 0) return 0;
-  if (OW == W && OH == H && ((uintptr_t)acc & 15u) == 0) {
+  const bool aligned = ((uintptr_t)acc & 15u) == 0;
+  const int nr = W * H;
+  if (OW == W && OH == H && aligned) {
     hipLaunchKernelGGL(sr_resolve_tile_kernel, dim3((n + 255) / 256), dim3(256), (size_t)256 * B * sizeof(float), st,
                        acc, n, B, p, out);
+  } else if (tmp && aligned) {
+    // upsampled display: resolve each render pixel once, then gather
+    hipLaunchKernelGGL(sr_resolve_tile_kernel, dim3((nr + 255) / 256), dim3(256), (size_t)256 * B * sizeof(float), st,
+                       acc, nr, B, p, tmp);
+    hipLaunchKernelGGL(sr_upsample_rgb_kernel, dim3((n + 255) / 256), dim3(256), 0, st, tmp, W, H, out, OW, OH);
   } else {
     hipLaunchKernelGGL(sr_resolve_kernel, dim3((n + 255) / 256), dim3(256), 0, st, acc, W, H, B, p, out, OW, OH);
   }

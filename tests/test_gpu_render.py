@@ -138,7 +138,10 @@ def test_running_mean_blocks_split(box_small, gpu):
     assert np.allclose(a.cpu().numpy(), r.accum.cpu().numpy(), rtol=1e-5, atol=1e-7)
 
 
-@pytest.mark.parametrize("W,H,OW,OH", [(10, 6, 20, 12), (37, 29, 37, 29)])  # upsampled; tiled (partial tile)
+# x2 upsampled (the reference's RESOLUTION_SCALE, main.py:16); a ratio that
+# is not an integer, with the render grid larger than one tile; the display
+# equal to the render grid (a partial last tile)
+@pytest.mark.parametrize("W,H,OW,OH", [(10, 6, 20, 12), (23, 17, 61, 40), (37, 29, 37, 29)])
 def test_resolve_matches_color_rays_closed_form(box_small, gpu, W, H, OW, OH):
     import torch
 
