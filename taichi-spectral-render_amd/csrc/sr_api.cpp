@@ -779,9 +779,16 @@ int sr_render_normals(sr_handle* h, int width, int height, int out_w, int out_h,
   if (width <= 0 || height <= 0 || out_w <= 0 || out_h <= 0) return fail(SR_EINVAL, "bad size");
   if ((long long)out_w * out_h >= (1LL << 31)) return fail(SR_EINVAL, "display grid too large");
   HIPCHK(hipSetDevice(h->device));
+  if ((long long)width * height >= (1LL << 31)) return fail(SR_EINVAL, "render grid too large");
   SceneDev s;
   fill_scene_dev(h, s);
-  if (sr_launch_normals(s, width, height, out_w, out_h, dev_rgb, (uint32_t*)h->overflow.p, h->st))
+  float* tmp = nullptr;
+  if (out_w != width || out_h != height) {
+    int rc = h->rgb_tmp.ensure((size_t)width * height * 3 * sizeof(float));
+    if (rc) return rc;
+    tmp = (float*)h->rgb_tmp.p;
+  }
+  if (sr_launch_normals(s, width, height, out_w, out_h, dev_rgb, tmp, (uint32_t*)h->overflow.p, h->st))
     return fail(SR_EHIP, "normals launch failed");
   return 0;
 }

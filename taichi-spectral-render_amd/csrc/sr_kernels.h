@@ -104,6 +104,7 @@ int sr_launch_fill(float* p, size_t n, float v, hipStream_t st);
 // or null (then each display pixel resolves its render pixel itself).
 int sr_launch_resolve(const float* acc, int W, int H, int B, const ResolveParams& p, float* out, int OW, int OH,
                       float* tmp, hipStream_t st);
-int sr_launch_normals(const SceneDev& s, int W, int H, int OW, int OH, float* out, uint32_t* overflow,
+// tmp: W * H * 3 floats of scratch for an upsampled display, or null.
+int sr_launch_normals(const SceneDev& s, int W, int H, int OW, int OH, float* out, float* tmp, uint32_t* overflow,
                       hipStream_t st);
 int sr_launch_rgba8(const float* rgb, size_t n, void* out, hipStream_t st);

@@ -116,7 +116,8 @@ __global__ void sr_upsample_rgb_kernel(const float* __restrict__ rgb, int W, int
 // visualize_normals, main.py:1632-1660: a pixel-centre camera ray per render
 // pixel, colour = shading normal * 0.5 + 0.5 (black on a miss), then the
 // same nearest upsample to the display grid as color_rays.  One thread per
-// display pixel.
+// display pixel (the launcher runs it on the render grid and gathers when
+// the display is larger).
 __global__ __launch_bounds__(256) void sr_normals_kernel(SceneDev s, int W, int H, int OW, int OH,
                                                          float* __restrict__ out, uint32_t* overflow) {
   __shared__ int lds_stack[SR_STACK_SIZE * 256];
@@ -157,12 +158,22 @@ __global__ void sr_rgba8_kernel(const float* __restrict__ rgb, size_t n, uchar4*
 }
 
 // ====================================================================== launchers
-int sr_launch_normals(const SceneDev& s, int W, int H, int OW, int OH, float* out, uint32_t* overflow,
+int sr_launch_normals(const SceneDev& s, int W, int H, int OW, int OH, float* out, float* tmp, uint32_t* overflow,
                       hipStream_t st) {
   const long long n = (long long)OW * OH;
   if (n <= 0) return 0;
-  hipLaunchKernelGGL(sr_normals_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, s, W, H, OW, OH, out,
-                     overflow);
+  if ((OW != W || OH != H) && tmp) {
+    // upsampled display: one ray per render pixel (pixels_geometry,
+    // main.py:1633-1653), then the nearest gather (main.py:1655-1658)
+    const long long nr = (long long)W * H;
+    hipLaunchKernelGGL(sr_normals_kernel, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, st, s, W, H, W, H, tmp,
+                       overflow);
+    hipLaunchKernelGGL(sr_upsample_rgb_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, tmp, W, H, out,
+                       OW, OH);
+  } else {
+    hipLaunchKernelGGL(sr_normals_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, s, W, H, OW, OH, out,
+                       overflow);
+  }
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

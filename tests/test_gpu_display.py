@@ -46,6 +46,10 @@ def test_normals_view_matches_brute_force(box, gpu):
     assert close.mean() >= 0.995
     miss = ~hit.reshape(W, H)[ri][:, rj]
     assert (img[miss] == 0).all()
+    # the upsampled view is the render-grid view gathered, bit for bit (one
+    # ray per render pixel, as the reference's pixels_geometry)
+    same = r.visualize_normals(W, H).cpu().numpy()
+    assert np.array_equal(img, same[ri][:, rj])
 
 
 def test_rgba8_closed_form(box, gpu):

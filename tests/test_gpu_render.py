@@ -166,6 +166,12 @@ def test_resolve_matches_color_rays_closed_form(box_small, gpu, W, H, OW, OH):
     fin = np.isfinite(ldr)
     assert np.allclose(out[fin], ldr[fin], rtol=1e-4, atol=1e-5)
     assert np.array_equal(np.isnan(out), ~fin)
+    if (OW, OH) != (W, H):
+        # an upsampled display is the render-grid resolve gathered, bit for bit
+        base = r.color_rays(cs, exposure=1.0, gamma=2.0).cpu().numpy()
+        ri32 = (np.arange(OW, dtype=np.float32) / np.float32(OW) * np.float32(W)).astype(int)
+        rj32 = (np.arange(OH, dtype=np.float32) / np.float32(OH) * np.float32(H)).astype(int)
+        assert np.array_equal(out, base[ri32][:, rj32], equal_nan=True)
 
 
 def test_upload_rejects_bad_material_ids(gpu):
