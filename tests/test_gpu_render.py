@@ -174,6 +174,28 @@ def test_resolve_matches_color_rays_closed_form(box_small, gpu, W, H, OW, OH):
         assert np.array_equal(out, base[ri32][:, rj32], equal_nan=True)
 
 
+def test_upsampled_resolve_after_resizing(box_small, gpu):
+    """One handle, render grids of growing size: the display-size scratch
+    image grows with them and every upsampled resolve is the gathered
+    render-grid resolve."""
+    import torch
+
+    from spectral.data import srgb_colour_system
+
+    cfg, flat, _ = box_small
+    r = _renderer(flat, cfg, 8, 6, gpu)
+    cs = srgb_colour_system(cfg.wl_min, cfg.wl_max, cfg.bands)
+    rng = np.random.default_rng(1)
+    for W, H, OW, OH in ((8, 6, 16, 12), (20, 14, 40, 28), (9, 7, 30, 11)):
+        r.set_resolution(W, H)
+        r.accum.copy_(torch.from_numpy(rng.uniform(0, 2, (W, H, cfg.bands)).astype(np.float32)))
+        up = r.color_rays(cs, out_width=OW, out_height=OH).cpu().numpy()
+        base = r.color_rays(cs).cpu().numpy()
+        ri = (np.arange(OW, dtype=np.float32) / np.float32(OW) * np.float32(W)).astype(int)
+        rj = (np.arange(OH, dtype=np.float32) / np.float32(OH) * np.float32(H)).astype(int)
+        assert np.array_equal(up, base[ri][:, rj], equal_nan=True), (W, H, OW, OH)
+
+
 def test_upload_rejects_bad_material_ids(gpu):
     from spectral._lib import SpectralRenderError
     from spectral.render import Renderer
