@@ -312,6 +312,11 @@ def bench_rows(reps=5, log=print, only=None):
     ms_c = timed(lambda: r.to_rgba8(box["rgb"]))
     npx = 1024 * 1024
     out["normals_view"] = {"pixels": npx, "ms": round(ms_n, 3), "Mpixels_per_s": round(npx / ms_n / 1e3, 1)}
+    # the reference's default display, twice the render grid: one ray per
+    # render pixel, then the gather to 2048 x 2048
+    ms_n2 = timed(lambda: r.visualize_normals(2048, 2048))
+    out["normals_view_x2"] = {"render_pixels": npx, "display_pixels": 4 * npx, "ms": round(ms_n2, 3),
+                              "display_Mpixels_per_s": round(4 * npx / ms_n2 / 1e3, 1)}
     gbs = npx * 16 / ms_c / 1e6
     out["rgba8"] = {"pixels": npx, "ms": round(ms_c, 4), "GB_per_s": round(gbs, 1), "frac_of_8TBps": round(gbs / 8000, 3),
                     "note": "16 MB per frame: launch-bound at this size"}
