@@ -210,3 +210,27 @@ def test_bad_material_id_rejected(gpu, box_flat):
     broken = flat.__class__(**{**flat.__dict__, "tri_mat": bad})
     with pytest.raises(_lib.SpectralRenderError, match="material id"):
         Renderer(broken, cfg.bands, cfg.wl_min, cfg.wl_max, device=gpu, build="device")
+
+
+def test_scene_array_errors(box):
+    """sr_scene_array: a size query needs no buffer; a buffer smaller than
+    the array, an unknown array id and a handle without a scene are errors."""
+    import ctypes
+
+    _, host, _ = box
+    size = ctypes.c_size_t()
+    _lib.call("sr_scene_array", host._h, 0, None, 0, ctypes.byref(size))
+    assert size.value == host.scene_info()["bvh_nodes"] * 128
+    small = np.empty(size.value - 16, np.uint8)
+    with pytest.raises(_lib.SpectralRenderError, match="capacity"):
+        _lib.call("sr_scene_array", host._h, 0, small.ctypes.data_as(ctypes.c_void_p), small.nbytes,
+                  ctypes.byref(size))
+    with pytest.raises(_lib.SpectralRenderError, match="unknown scene array"):
+        _lib.call("sr_scene_array", host._h, 10, None, 0, ctypes.byref(size))
+    h = ctypes.c_void_p()
+    _lib.call("sr_create", 8, 400.0, 700.0, ctypes.byref(h))
+    try:
+        with pytest.raises(_lib.SpectralRenderError, match="ESTATE"):
+            _lib.call("sr_scene_array", h, 0, None, 0, ctypes.byref(size))
+    finally:
+        _lib.call("sr_destroy", h)
