@@ -53,3 +53,21 @@ def test_create_rejects_bad_bands_and_wavelengths():
         assert h.value is None
         msg = lib.sr_last_error().decode()
         assert ("bands" in msg) or ("wl_min" in msg)
+
+
+def test_every_entry_point_rejects_a_null_handle():
+    """Every entry point that takes a handle or a batch returns SR_EINVAL
+    with a message for a null one, before any device call (all other
+    arguments zero or null)."""
+    from spectral import _lib
+
+    lib = _lib.load()
+    skip = {"sr_last_error", "sr_version", "sr_create", "sr_destroy", "sr_batch_create", "sr_batch_destroy"}
+    for name, (argtypes, _) in _lib.SIGNATURES.items():
+        if name in skip:
+            continue
+        args = [0.0 if t is ctypes.c_float else (0 if t in (ctypes.c_int, ctypes.c_int64, ctypes.c_uint64,
+                                                            ctypes.c_uint32, ctypes.c_size_t) else None)
+                for t in argtypes]
+        assert getattr(lib, name)(*args) == -1, name
+        assert b"null" in lib.sr_last_error(), name
