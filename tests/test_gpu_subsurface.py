@@ -142,3 +142,21 @@ def test_walk_with_light_tree_same_expectation(gpu):
     m0, m1 = tot[False].mean(), tot[True].mean()
     print(f"tree / CDF with the walk: {m1 / m0:.5f} +- {se / m0:.5f}")
     assert abs(m1 - m0) < 4 * se + 5e-4 * m0
+
+
+@pytest.mark.gpu
+def test_walks_end_at_the_step_limit(gpu):
+    # w = 1 and sigma_s = 1000 (mean free path 1e-3) in a 5-unit slab: a
+    # walk enters 0.1 mean free paths deep, so most leave within a few
+    # steps, but the long tail passes 256 steps and is ended there; the
+    # render finishes, stays finite, and returns visibly less than the sky
+    # radiance 1 that unbounded walks would (measured 0.868)
+    flat = _scene(_box([-10, -5, -10], [10, 0, 10], 1), 1.0, sigma_s=1000.0)
+    r = Renderer(flat, B, 400.0, 700.0, width=16, height=16, device=gpu)
+    r.clear_pixel_space()
+    r.render(n_blocks=4, max_depth=5, seed=4, subsurface=True)
+    a = r.accum.cpu().numpy()
+    assert np.isfinite(a).all() and (a >= 0).all()
+    st = r.stats()
+    assert st["nonfinite"] == 0
+    assert 0.6 < float(a.mean()) < 0.95
