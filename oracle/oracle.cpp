@@ -922,7 +922,7 @@ static int lt_sample(const Scene& s, int band, V3 x, float u, float& p) {
   p = 1.0f;
   if (s.lt_nodes[0].right < 0) return 0;
   int f = s.lt_nodes[0].left;
-  for (;;) {
+  for (int level = 0; level < 64; ++level) {  // bounded whatever the arrays hold
     float il = lt_importance(s.lt_nodes[f], s.lt_power[(size_t)band * n + f], x);
     float ir = lt_importance(s.lt_nodes[f + 1], s.lt_power[(size_t)band * n + f + 1], x);
     float sum = il + ir;
@@ -942,12 +942,15 @@ static int lt_sample(const Scene& s, int band, V3 x, float u, float& p) {
     if (s.lt_nodes[c].right < 0) return c;
     f = s.lt_nodes[c].left;
   }
+  return -1;
 }
 
 static float lt_pdf(const Scene& s, int band, V3 x, int leaf) {
   const int n = (int)s.lt_nodes.size();
   float p = 1.0f;
+  int level = 0;
   for (int c = leaf; c != 0; c = s.lt_nodes[c].parent) {
+    if (++level > 64) return 0.0f;
     int f = c & ~1;
     float il = lt_importance(s.lt_nodes[f], s.lt_power[(size_t)band * n + f], x);
     float ir = lt_importance(s.lt_nodes[f + 1], s.lt_power[(size_t)band * n + f + 1], x);

@@ -510,7 +510,9 @@ __device__ int lt_sample(const SceneDev& s, int band, f3 x, float u, float& p) {
   p = 1.0f;
   if (s.n_lkeys == 1) return 0;  // the root is the only leaf
   int f = 2;                     // the root's children
-  for (;;) {
+  // median splits keep the depth below 32; the bound only guarantees that
+  // the loop ends whatever the arrays hold
+  for (int level = 0; level < 64; ++level) {
     const LightNode nl = s.lnodes[f], nr = s.lnodes[f + 1];
     const float2 P = lt_power_pair(s, band, f);
     const float il = lt_importance(nl, P.x, x), ir = lt_importance(nr, P.y, x);
@@ -536,6 +538,7 @@ __device__ int lt_sample(const SceneDev& s, int band, f3 x, float u, float& p) {
     if (right < 0) return node;
     f = next;
   }
+  return -1;
 }
 
 // Probability that lt_sample picks `leaf` from x: the walk up the parents,
@@ -543,7 +546,8 @@ __device__ int lt_sample(const SceneDev& s, int band, f3 x, float u, float& p) {
 __device__ float lt_pdf(const SceneDev& s, int band, f3 x, int leaf) {
   float p = 1.0f;
   int c = leaf;
-  while (c > 0) {
+  for (int level = 0; c > 0; ++level) {
+    if (level >= 64) return 0.0f;  // (bounded like lt_sample)
     const int f = c & ~1;
     const LightNode nl = s.lnodes[f], nr = s.lnodes[f + 1];
     const float2 P = lt_power_pair(s, band, f);
