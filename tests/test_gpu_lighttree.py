@@ -153,7 +153,7 @@ def test_sparse_lights_tree_same_expectation_far_lower_variance(gpu):
     assert ratio < 0.5
 
 
-@pytest.mark.parametrize("scene", ["sparse_lamps", "emitting_sphere"])
+@pytest.mark.parametrize("scene", ["sparse_lamps", "emitting_sphere", "one_lamp"])
 def test_tree_matched_seed_parity_with_oracle(gpu, scene):
     """The oracle's light-tree NEE (its own restatement of the descent and of
     the pdf walk, from the angles themselves rather than the kernel's
@@ -162,11 +162,14 @@ def test_tree_matched_seed_parity_with_oracle(gpu, scene):
     99.7% of the pixel-bands and a mean relative error below 1e-4 (the two
     follow the same paths; they part only where f32 rounding of the
     importance flips a pick).  Scenes: the sparse lamps, where the tree's
-    picks and MIS weights differ most from the CDF's, and the box with its
+    picks and MIS weights differ most from the CDF's, the box with its
     icosphere emitting in bands >= 8 only (per-band powers with empty
-    subtrees, cones of every width, glossy walls and a dielectric)."""
-    if scene == "sparse_lamps":
-        cfg = configs.lamps(n=16, extent=16.0)
+    subtrees, cones of every width, glossy walls and a dielectric), and a
+    single lamp (two leaves under the root)."""
+    if scene in ("sparse_lamps", "one_lamp"):
+        # one lamp = two emitting triangles: the smallest inner tree (the
+        # root's children are both leaves)
+        cfg = configs.lamps(n=16, extent=16.0) if scene == "sparse_lamps" else configs.lamps(n=1)
         flat = flatten_scene(cfg.scene, cfg.bands, cfg.wl_min, cfg.wl_max)
         depth = 4
     else:
