@@ -40,15 +40,21 @@ __device__ __forceinline__ float unord_f(unsigned u) {
   return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
 }
 
-// bounds[0..2] centroid min, [3..5] centroid max (ordered uints), [6] max |coord|
-__global__ void k_faces(const float* __restrict__ v, const float* __restrict__ nrm, const int* __restrict__ mat,
-                        int n, int n_mats, float4* __restrict__ shade, float* __restrict__ area,
-                        float* __restrict__ box, float* __restrict__ cen, unsigned* __restrict__ bounds,
-                        unsigned* __restrict__ bad) {
+// bounds[0..2] centroid min, [3..5] centroid max (ordered uints), [6] max |coord|.
+// Grid-stride over the faces with at most kFaceBlocks workgroups: the
+// centroid bounds are reduced per wave, then per workgroup in LDS, so the
+// 7 global atomics are issued once per workgroup (issued once per wave,
+// the shared addresses serialise thousands of them).  Min and max are
+// exact, so the result does not depend on the reduction order.
+constexpr int kFaceBlocks = 1024;
+__global__ __launch_bounds__(256) void k_faces(const float* __restrict__ v, const float* __restrict__ nrm,
+                                               const int* __restrict__ mat, int n, int n_mats,
+                                               float4* __restrict__ shade, float* __restrict__ area,
+                                               float* __restrict__ box, float* __restrict__ cen,
+                                               unsigned* __restrict__ bounds, unsigned* __restrict__ bad) {
 #pragma clang fp contract(off)
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
   float cmin[3] = {3.0e38f, 3.0e38f, 3.0e38f}, cmax[3] = {-3.0e38f, -3.0e38f, -3.0e38f}, mabs = 1e-3f;
-  if (f < n) {
+  for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < n; f += gridDim.x * blockDim.x) {
     const float* p = v + (size_t)f * 9;
     const float* q = nrm + (size_t)f * 9;
     const float e1x = p[3] - p[0], e1y = p[4] - p[1], e1z = p[5] - p[2];
@@ -74,8 +80,8 @@ __global__ void k_faces(const float* __restrict__ v, const float* __restrict__ n
       box[(size_t)f * 6 + 3 + ax] = hi;
       const float c = (p[ax] + p[3 + ax] + p[6 + ax]) * (1.0f / 3.0f);
       cen[(size_t)f * 3 + ax] = c;
-      cmin[ax] = c;
-      cmax[ax] = c;
+      cmin[ax] = fminf(cmin[ax], c);
+      cmax[ax] = fmaxf(cmax[ax], c);
       mabs = fmaxf(mabs, fmaxf(fabsf(lo), fabsf(hi)));
     }
   }
@@ -86,7 +92,25 @@ __global__ void k_faces(const float* __restrict__ v, const float* __restrict__ n
     }
     mabs = fmaxf(mabs, __shfl_xor(mabs, off));
   }
+  __shared__ float red[4][7];
+  const int w = threadIdx.x >> 6;
   if (__lane_id() == 0) {
+    for (int ax = 0; ax < 3; ++ax) {
+      red[w][ax] = cmin[ax];
+      red[w][3 + ax] = cmax[ax];
+    }
+    red[w][6] = mabs;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = (int)(blockDim.x + 63) >> 6;
+    for (int k = 1; k < nw; ++k) {
+      for (int ax = 0; ax < 3; ++ax) {
+        cmin[ax] = fminf(cmin[ax], red[k][ax]);
+        cmax[ax] = fmaxf(cmax[ax], red[k][3 + ax]);
+      }
+      mabs = fmaxf(mabs, red[k][6]);
+    }
     for (int ax = 0; ax < 3; ++ax) {
       atomicMin(bounds + ax, ord_f(cmin[ax]));
       atomicMax(bounds + 3 + ax, ord_f(cmax[ax]));
@@ -974,7 +998,7 @@ int sr_build_scene_device(const float* tris_v, const float* tris_n, const int* t
     BCHK(hipMemcpyAsync(bounds, init, sizeof(init), hipMemcpyHostToDevice, st));
   }
   if (n > 0)
-    hipLaunchKernelGGL(k_faces, dim3(blocks(n)), dim3(256), 0, st, tris_v, tris_n, tri_mat, n, n_mats,
+    hipLaunchKernelGGL(k_faces, dim3(std::min(blocks(n), kFaceBlocks)), dim3(256), 0, st, tris_v, tris_n, tri_mat, n, n_mats,
                        (float4*)out->shade, area, fbox, cen, bounds, bounds + 7);
   BCHK(hipGetLastError());
   unsigned hb[8];
