@@ -606,7 +606,17 @@ struct NEE {  // main.py:355-359
 
 struct Counters {
   uint64_t segments = 0, traversals = 0, nonfinite = 0;
+  // Accounting for the HIP kernel's shadow-walk culling (not part of the
+  // reference semantics, which are executed in full here): shadow
+  // traversals whose result cannot reach the image -- the walk of a path
+  // with throughput 0, and every step that starts in a dense medium
+  // (sigma_t * EPSILON >= 14.5, so exp(-sigma_t t) < 1e-6 for any hit) with
+  // the light not yet passed -- and the non-finite NEE terms among those
+  // walks' contributions (dropped either way).  The kernel does not trace
+  // them; the tests compare its counters with these.
+  uint64_t culled = 0, culled_nf = 0;
 };
+static inline bool shadow_dense(float sigma_t) { return sigma_t * EPSILON >= 14.5f; }
 
 // Oren-Nayar factor shared by scatter_surface and evaluate_bsdf
 static inline float oren_nayar(float roughness, float NdotL, float NdotV, V3 l, V3 v, V3 n) {
@@ -854,18 +864,20 @@ static void evaluate_bsdf(const Scene& s, int vol_id, int surf_id, V3 V, V3 L, V
 
 // shadow_transmittance, main.py:1446-1499
 static float shadow_transmittance(const Scene& s, V3 origin, V3 dir, float max_dist, int tri_idx, int start_vol, int b,
-                                  Counters& cnt) {
+                                  Counters& cnt, bool cull = false) {
   float total = 1.0f;
   float traveled = 0.0f;
   V3 current_pos = origin;
   int vol = start_vol;
   V3 spos = current_pos, sdir = dir;
   for (int i = 0; i < SCATTER_BOUNCES; ++i) {
-    Hit hit = trace(s, spos, sdir);
-    cnt.traversals++;
-    float dts = hit.t;
     Mat vm = get_mat(s, vol, b);
     float absorb = (4.0f * PI_F * vm.k) / lam_m(s, b);
+    if (max_dist - traveled >= 0.0f && shadow_dense(absorb + vm.ss)) cull = true;
+    Hit hit = trace(s, spos, sdir);
+    cnt.traversals++;
+    if (cull) cnt.culled++;
+    float dts = hit.t;
     total *= std::exp(-(absorb + vm.ss) * hit.t);
     if (hit.t >= (max_dist - traveled) || hit.tri_idx == tri_idx) {
       total *= std::exp(-(absorb + vm.ss) * (max_dist - traveled));
@@ -991,7 +1003,7 @@ static NEE sample_direct_light(const Scene& s, const Ray& ray, V3 normVec, int s
     V3 pl = lt.v0 + bx * (lt.v1 - lt.v0) + by * (lt.v2 - lt.v0);
     V3 dl = normalize(pl - p);
     float dist = length(pl - p);
-    float tr = shadow_transmittance(s, p, dl, dist, lt_idx, ray.vol_mat_id, wl, cnt);
+    float tr = shadow_transmittance(s, p, dl, dist, lt_idx, ray.vol_mat_id, wl, cnt, ray.throughput == 0.0f);
     if (tr > 1e-6f) {
       V3 ln = lt.normal;
       float cl = tmax(0.0f, dot(ln, -dl));
@@ -1045,6 +1057,7 @@ static void cast_ray_path(const Scene& s, int W, int H, int i, int j, int band, 
     // make every later contribution NaN; the path ends here instead.
     if (!std::isfinite(ray.throughput)) { cnt.nonfinite++; break; }
     float direct = 0.0f, indirect = 0.0f;
+    bool nee_tp0 = false;  // this segment's NEE ran with throughput 0 (culled by the kernel)
     VolumeEvent ev = scatter_volume(s, ray, rng, cnt);
     cnt.segments++;
     if (ev.travel_distance < 0.0f) break;
@@ -1075,6 +1088,7 @@ static void cast_ray_path(const Scene& s, int W, int H, int i, int j, int band, 
       } else {
         NEE ns = sample_direct_light(s, ray, ev.shading_normal, ev.hit_mat_id, rng, cnt);
         nee_pos = ray.position;
+        nee_tp0 = ray.throughput == 0.0f;
         direct = ray.throughput * ns.emission * ns.contribution_factor * ns.weight;
         ray.path_length += 1;
         SurfaceEvent se = scatter_surface(s, ray, ev, rng);
@@ -1089,6 +1103,7 @@ static void cast_ray_path(const Scene& s, int W, int H, int i, int j, int band, 
     } else {
       NEE ns = sample_direct_light(s, ray, ev.shading_normal, ev.hit_mat_id, rng, cnt);
       nee_pos = ray.position;
+      nee_tp0 = ray.throughput == 0.0f;
       direct = ray.throughput * ns.emission * ns.contribution_factor * ns.weight;
       if (ev.hit_action == 2) {
         ray.direction = ev.direction;
@@ -1101,7 +1116,12 @@ static void cast_ray_path(const Scene& s, int W, int H, int i, int j, int band, 
       }
     }
     if (std::isfinite(indirect)) acc[active_field] += indirect; else cnt.nonfinite++;
-    if (std::isfinite(direct)) acc[active_field] += direct; else cnt.nonfinite++;
+    if (std::isfinite(direct)) {
+      acc[active_field] += direct;
+    } else {
+      cnt.nonfinite++;
+      if (nee_tp0) cnt.culled_nf++;
+    }
   }
 }
 
@@ -1243,13 +1263,14 @@ void or_trace(void* p, int n, const float* orig, const float* dir, float* out_t,
 // columns [i0, i1) and updates the running mean accum[W][H][B] in place
 // (average_and_reset_block main.py:1592-1607 applied once per block).
 // RNG key per path: c0 = i*H + j, c1 = block, c2 = band | (stream << 8).
-// stats = {segments, traversals, nonfinite}.
+// stats = {segments, traversals, nonfinite, culled shadow traversals, culled
+// non-finite NEE terms} (the last two: see Counters).
 int or_render(void* p, int W, int H, uint64_t seed, uint32_t stream, int first_block, int n_blocks, int depth,
               int i0, int i1, float* accum, double* stats) {
   const Scene& s = *(Scene*)p;
   const int B = s.B;
-  uint64_t seg = 0, trav = 0, nf = 0;
-#pragma omp parallel for schedule(dynamic, 1) reduction(+ : seg, trav, nf)
+  uint64_t seg = 0, trav = 0, nf = 0, cul = 0, cnf = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : seg, trav, nf, cul, cnf)
   for (int i = i0; i < i1; ++i) {
     std::vector<float> blk(B);
     Counters cnt;
@@ -1269,8 +1290,12 @@ int or_render(void* p, int W, int H, uint64_t seed, uint32_t stream, int first_b
       }
     }
     seg += cnt.segments; trav += cnt.traversals; nf += cnt.nonfinite;
+    cul += cnt.culled; cnf += cnt.culled_nf;
   }
-  if (stats) { stats[0] = (double)seg; stats[1] = (double)trav; stats[2] = (double)nf; }
+  if (stats) {
+    stats[0] = (double)seg; stats[1] = (double)trav; stats[2] = (double)nf;
+    stats[3] = (double)cul; stats[4] = (double)cnf;
+  }
   return 0;
 }
 

@@ -118,10 +118,11 @@ class OracleScene:
         if accum is None:
             accum = np.zeros((width, height, B), np.float32)
         i0, i1 = (0, width) if cols is None else cols
-        stats = np.zeros(3, np.float64)
+        stats = np.zeros(5, np.float64)
         load().or_render(self.h, width, height, int(seed), int(stream), int(first_block), int(n_blocks), int(depth),
                          int(i0), int(i1), _p(accum), _p(stats))
-        return accum, {"segments": int(stats[0]), "traversals": int(stats[1]), "nonfinite": int(stats[2])}
+        return accum, {"segments": int(stats[0]), "traversals": int(stats[1]), "nonfinite": int(stats[2]),
+                       "culled_traversals": int(stats[3]), "culled_nonfinite": int(stats[4])}
 
     def shadow(self, o, d, max_dist, tri, vol, band):
         return float(load().or_shadow(self.h, _p(_f32(o)), _p(_f32(d)), float(max_dist), int(tri), int(vol),

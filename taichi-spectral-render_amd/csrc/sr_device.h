@@ -237,8 +237,18 @@ __device__ __forceinline__ void visit_leaf(const float4* __restrict__ tris, int 
 // t < t_init are reported (t_init = 1e9, the reference's miss distance,
 // for a plain closest-hit query); with no such hit the result is
 // {t_init, -1}.
+//
+// t_near < t_init asks a narrower question: the closest hit with
+// t <= t_near exactly, and otherwise only whether any hit with t < t_init
+// exists (then the result is {nextafter(t_near), -2}).  This is what a
+// path segment in a medium needs once its free-flight distance d is drawn:
+// a surface farther than d + EPSILON is never reached, but whether one
+// exists before ESCAPE still decides between a volume event and an escape
+// (main.py:1210-1218).  The first far hit found shrinks the search to
+// [0, t_near], so a dense medium's traversal stops almost at once.
 __device__ __forceinline__ HitD trace_closest(const Node4* __restrict__ nodes, const float4* __restrict__ tris, f3 o,
-                                              f3 d, int* stk, int stride, uint32_t* overflow, float t_init = 1e9f) {
+                                              f3 d, int* stk, int stride, uint32_t* overflow, float t_init = 1e9f,
+                                              float t_near = INFINITY) {
   HitD h;
   h.t = t_init;  // HitRecord(t=1e9), main.py:833
   h.prim = INT32_MIN;
@@ -253,6 +263,8 @@ __device__ __forceinline__ HitD trace_closest(const Node4* __restrict__ nodes, c
   int sp = 0;
   int cur = 0;       // root node; SR_EMPTY = nothing left
   int leaf = SR_EMPTY;  // parked leaf
+  const float t_near_up = nextafterf(t_near, INFINITY);
+  bool far = false;  // a hit in (t_near, t_init) was seen
   while (cur != SR_EMPTY || leaf != SR_EMPTY) {
     // node loop
     while (cur != SR_EMPTY && cur >= 0) {
@@ -270,6 +282,11 @@ __device__ __forceinline__ HitD trace_closest(const Node4* __restrict__ nodes, c
     // leaf loop
     while (leaf != SR_EMPTY) {
       visit_leaf(tris, leaf, o, d, h);
+      if (h.t > t_near && h.prim != INT32_MIN) {  // far hit: from now on only t <= t_near counts
+        far = true;
+        h.t = t_near_up;
+        h.prim = INT32_MIN;
+      }
       leaf = SR_EMPTY;
       if (cur != SR_EMPTY && cur < 0) {
         leaf = cur;
@@ -277,7 +294,7 @@ __device__ __forceinline__ HitD trace_closest(const Node4* __restrict__ nodes, c
       }
     }
   }
-  if (h.prim == INT32_MIN) h.prim = -1;
+  if (h.prim == INT32_MIN) h.prim = far ? -2 : -1;
   return h;
 }
 

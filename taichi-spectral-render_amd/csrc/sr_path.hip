@@ -572,14 +572,28 @@ __device__ __forceinline__ int lt_leaf_of(const SceneDev& s, int face) {
 
 // sample_direct_light, main.py:1296-1326, up to the shadow walk: pick a
 // light and a point on it with the uniform block (seg, slot).
+//
+// The walk is not traced when its result cannot reach the image: with a
+// throughput of exactly 0 the contribution tp * emission * factor * weight
+// is 0 (or NaN, which is dropped) whatever the transmittance; and from a
+// medium whose sigma_t = 4 pi k / lambda + sigma_s is dense (shadow_dense)
+// the first step's attenuation exp(-sigma_t t), t > EPSILON, is below the
+// 1e-6 that sample_direct_light requires (main.py:1329).  sigma_vol is the
+// path medium's sigma_t in the active band (0 when unknown).
+__device__ __forceinline__ bool shadow_dense(float sigma_t) {
+  // exp(-sigma_t * t) <= exp(-14.5) = 5.0e-7 < 1e-6 for every t > EPSILON
+  return sigma_t * SR_EPSILON >= 14.5f;
+}
+
 template <bool ADJ>
-__device__ __forceinline__ void nee_setup(Lane& L, const Ctx<ADJ>& c, int surf, int seg, int slot) {
+__device__ __forceinline__ void nee_setup(Lane& L, const Ctx<ADJ>& c, int surf, int seg, int slot,
+                                          float sigma_vol) {
   const Job& a = c.a;
   const SceneDev& s = a.s;
   const int ab = L.active;
   L.nee_surf = surf;
   const int nl = s.light_count[ab];
-  if (nl <= 0) {
+  if (nl <= 0 || L.tp == 0.0f || shadow_dense(sigma_vol)) {
     L.fdirect = 0.0f;
     L.sdir = L.pos;
     L.phase = PH_FIN;
@@ -663,7 +677,7 @@ __device__ __forceinline__ void walk_step(Lane& L, const Ctx<ADJ>& c, const HitD
     L.efront = 1;
     L.eaction = 1;
     L.walk = 2;
-    nee_setup<ADJ>(L, c, -2, (L.seg << 9) | L.wstep, SR_SLOT_WALK_NEE);
+    nee_setup<ADJ>(L, c, -2, (L.seg << 9) | L.wstep, SR_SLOT_WALK_NEE, 0.0f);
     return;
   }
   L.pos = L.pos + L.dir * L.smax;
@@ -676,6 +690,20 @@ __device__ __forceinline__ void walk_step(Lane& L, const Ctx<ADJ>& c, const HitD
   L.dir = hg_sample(L.dir, L.soff, u.u[1], u.u[2]);
   L.smax = walk_flight(u.u[0], L.stot);
   L.phase = PH_EXT;
+}
+
+// Free-flight distance of scatter_volume (main.py:1203-1204), drawn from the
+// segment's EXT block before its trace: the trace then needs the exact
+// closest hit only up to dvol + EPSILON (trace_closest's t_near).
+template <bool ADJ>
+__device__ __forceinline__ float ext_flight(const Lane& L, const Ctx<ADJ>& c, float& sa, MatRec& vm, U4& re4) {
+  const Job& a = c.a;
+  const SceneDev& s = a.s;
+  const int ab = L.active;
+  vm = get_mat(s, L.vol, ab);
+  sa = fdiv(4.0f * SR_PI * vm.k, s.lam_m[ab]);
+  re4 = draws(a, L, L.seg, SR_SLOT_EXT);
+  return fdiv(-logf(-re4.u[0] + 1.0f - 1e-7f), vm.ss + sa + 1e-6f);
 }
 
 // Extension ray came back: scatter_volume (main.py:1190-1292) and the
@@ -691,10 +719,12 @@ __device__ __forceinline__ void on_extension(Lane& L, const Ctx<ADJ>& c, const H
   L.n_seg++;
   L.n_trav++;
   const int ab = L.active;
-  MatRec vm = get_mat(s, L.vol, ab);
-  float sa = fdiv(4.0f * SR_PI * vm.k, s.lam_m[ab]);
-  const U4 re4 = draws(a, L, L.seg, SR_SLOT_EXT);
-  float dvol = fdiv(-logf(-re4.u[0] + 1.0f - 1e-7f), vm.ss + sa + 1e-6f);
+  MatRec vm;
+  float sa;
+  U4 re4;
+  const float dvol = ext_flight<ADJ>(L, c, sa, vm, re4);
+  // h is exact for t <= dvol + EPSILON; beyond that only "some hit before
+  // ESCAPE" (prim -2, t just above dvol + EPSILON) or "none" (t = ESCAPE)
   float ht = h.t;
   f3 hn = mk(0.f, 0.f, 0.f), hsn = mk(0.f, 0.f, 0.f);
   int hfront = 0, hmat = 0;
@@ -837,7 +867,7 @@ __device__ __forceinline__ void on_extension(Lane& L, const Ctx<ADJ>& c, const H
     }
     surf = L.emat;
   }
-  nee_setup<ADJ>(L, c, surf, L.seg, SR_SLOT_NEE);
+  nee_setup<ADJ>(L, c, surf, L.seg, SR_SLOT_NEE, sa + vm.ss);
 }
 
 // Shadow ray came back: one step of shadow_transmittance (main.py:1456-1498);
@@ -877,7 +907,19 @@ __device__ __forceinline__ void on_shadow(Lane& L, const Ctx<ADJ>& c, const HitD
       L.soff = h.t + SR_EPSILON;
       L.svol = sh.mat * sh.front;
       L.sstep += 1;
-      if (L.sstep >= SR_SCATTER_BOUNCES) done = true;
+      if (L.sstep >= SR_SCATTER_BOUNCES) {
+        done = true;
+      } else if (L.smax - L.strav >= 0.0f && !(ADJ && L.svol == c.adj.sig_mat)) {
+        // The next step would start in medium svol.  If it is dense, that
+        // step's exp(-sigma_t t) (t > EPSILON) and, when the light is
+        // reached, exp(-sigma_t (smax - strav)) <= 1 leave stot <= 5e-7: no
+        // contribution either way (main.py:1463-1469, 1329), so it is not traced.
+        const MatRec nv = get_mat(s, L.svol, ab);
+        if (shadow_dense(fdiv(4.0f * SR_PI * nv.k, s.lam_m[ab]) + nv.ss)) {
+          L.stot = 0.0f;
+          done = true;
+        }
+      }
     }
   }
   if (!done) return;
@@ -1073,8 +1115,18 @@ __device__ __forceinline__ void path_loop(const PathArgs& P, int* stk, float* ld
     if (L.phase == (do_ext ? PH_EXT : PH_SHD)) {
       f3 ro = do_ext ? L.pos : L.pos + L.sdir * L.soff;
       f3 rd = do_ext ? L.dir : L.sdir;
-      float t_init = 1e9f;
-      if (SSS && do_ext && L.walk == 1) t_init = nextafterf(L.smax, INFINITY);  // hits at t <= the flight
+      float t_init = 1e9f, t_near = INFINITY;
+      if (SSS && do_ext && L.walk == 1) {
+        t_init = nextafterf(L.smax, INFINITY);  // hits at t <= the flight
+      } else if (do_ext) {
+        // a path segment: hits at or beyond ESCAPE mean "escaped"
+        // (main.py:1210), and past the free flight only their existence counts
+        MatRec vm;
+        float sa;
+        U4 re4;
+        t_init = SR_ESCAPE;
+        t_near = ext_flight<ADJ>(L, c, sa, vm, re4) + SR_EPSILON;
+      }
       if (!do_ext) {
         // A shadow step through a medium with sigma_a + sigma_s = 0 needs
         // only the hits before the light: any hit at t >= smax - strav ends
@@ -1087,7 +1139,7 @@ __device__ __forceinline__ void path_loop(const PathArgs& P, int* stk, float* ld
       }
       const Node4* nodes = J.s.nodes;
       const float4* tris = J.s.tris;
-      HitD h = trace_closest(nodes, tris, ro, rd, stk, stride, P.overflow, t_init);
+      HitD h = trace_closest(nodes, tris, ro, rd, stk, stride, P.overflow, t_init, t_near);
       if (do_ext) on_extension<ADJ, SSS>(L, c, h);
       else on_shadow<ADJ, SSS>(L, c, h);
     }

@@ -77,8 +77,13 @@ def test_counters_match_cpu(box_small, gpu):
     r.render(n_blocks=16, max_depth=cfg.depth, seed=cfg.seed)
     st = r.stats()
     assert abs(st["segments"] - st0["segments"]) <= 0.005 * st0["segments"]
-    assert abs(st["traversals"] - st0["traversals"]) <= 0.005 * st0["traversals"]
-    assert abs(st["nonfinite"] - st0["nonfinite"]) <= 0.02 * max(st0["nonfinite"], 50)
+    # the kernel does not trace shadow walks that cannot contribute (the
+    # oracle traces them and counts them as culled)
+    assert st0["culled_traversals"] > 0
+    traced = st0["traversals"] - st0["culled_traversals"]
+    assert abs(st["traversals"] - traced) <= 0.005 * traced
+    nf = st0["nonfinite"] - st0["culled_nonfinite"]
+    assert abs(st["nonfinite"] - nf) <= 0.02 * max(nf, 50)
 
 
 def test_independent_streams_statistical(box_small, gpu):
