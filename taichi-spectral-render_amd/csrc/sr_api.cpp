@@ -80,7 +80,12 @@ struct sr_handle {
   int device = 0;
   hipStream_t own = nullptr, st = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev_switch = nullptr;  // orders work across a change of stream (sr_set_stream)
   bool timed = false;
+  // Bumped whenever something a launched job table captures changes: the
+  // scene arrays (upload), the camera, the material tables, the block-sum
+  // buffer.  A batch re-fills its job table when a handle's generation moved.
+  uint64_t gen = 0;
   // scene
   bool has_scene = false;
   int n_tris = 0, n_mats = 0, n_eem = 0, n_nodes = 0, depth = 0, max_lights = 0, n_slots = 0;
@@ -90,6 +95,7 @@ struct sr_handle {
   DevBuf rgb_tmp;                // resolved render-grid image for an upsampled display
   std::vector<float> h_light_area;
   std::vector<int> h_light_count;
+  std::vector<unsigned char> h_se;  // surface_emission [M][B] the light lists were built from
   float cam_o[3], cam_u[3], cam_v[3], cam_w[3];
   // work
   DevBuf sum, work, jobs, overflow, tmp, alb_off;
@@ -138,6 +144,13 @@ static void set_camera_host(sr_handle* h, const float* cam) {
   h->cam_w[0] = w.x; h->cam_w[1] = w.y; h->cam_w[2] = w.z;
 }
 
+static std::vector<unsigned char> surface_emission_flags(const int* mat_i32, int n_mats, int B) {
+  std::vector<unsigned char> se((size_t)n_mats * B);
+  for (int m = 0; m < n_mats; ++m)
+    for (int b = 0; b < B; ++b) se[(size_t)m * B + b] = mat_i32[((size_t)m * 2 + 1) * B + b] ? 1 : 0;
+  return se;
+}
+
 static int upload_materials(sr_handle* h, const float* mat_f32, const int* mat_i32, const int* mat_eem, int n_mats,
                             const float* eem, int n_eem) {
   const int B = h->B;
@@ -184,7 +197,8 @@ struct sr_batch {
     uint64_t seed;
     uint32_t stream;
     float* accum;
-    int flags;  // SR_RENDER_*
+    int flags;     // SR_RENDER_*
+    uint64_t gen;  // h->gen when the job table was last filled
   };
   std::vector<Spec> specs;
   std::vector<Job> host_jobs;
@@ -215,6 +229,13 @@ static WorkView work_view(void* base, int n) {
   w.counters = w.clocks + (n + 1);
   return w;
 }
+// A job's work queue is a u32 that every wave advances by `chunk` per claim,
+// with at most one claim per wave past the end: the counter must not wrap.
+static bool queue_overflows(uint32_t n_items, int grid, int block, uint32_t chunk) {
+  const unsigned long long waves = ((unsigned long long)grid * (unsigned long long)block + 63) / 64;
+  return (unsigned long long)n_items + (waves + 1) * (unsigned long long)chunk >= (1ull << 32);
+}
+
 // Graph-safe reset of the work area (kernels only, no memset nodes).
 static int reset_work(void* base, int n, hipStream_t st) {
   WorkView w = work_view(base, n);
@@ -245,6 +266,7 @@ int sr_create(int bands, float wl_min, float wl_max, sr_handle** out) {
   if (e == hipSuccess) e = hipStreamCreate(&h->own);  // blocking: ordered with the null stream
   if (e == hipSuccess) e = hipEventCreate(&h->ev0);
   if (e == hipSuccess) e = hipEventCreate(&h->ev1);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_switch, hipEventDisableTiming);
   if (e != hipSuccess) {
     delete h;
     return fail(SR_EHIP, "HIP init failed: %s", hipGetErrorString(e));
@@ -277,6 +299,7 @@ int sr_destroy(sr_handle* h) {
   for (DevBuf* b : bufs) b->release();
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->ev_switch) (void)hipEventDestroy(h->ev_switch);
   if (h->own) (void)hipStreamDestroy(h->own);
   delete h;
   return 0;
@@ -284,7 +307,15 @@ int sr_destroy(sr_handle* h) {
 
 int sr_set_stream(sr_handle* h, void* stream) {
   if (!h) return fail(SR_EINVAL, "null handle");
-  h->st = stream ? (hipStream_t)stream : h->own;
+  hipStream_t ns = stream ? (hipStream_t)stream : h->own;
+  if (ns == h->st) return 0;
+  // Everything already enqueued on the old stream (it shares the handle's
+  // block sums, work counters and staging buffers) completes before
+  // anything enqueued on the new one: an event, no host wait.
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipEventRecord(h->ev_switch, h->st));
+  HIPCHK(hipStreamWaitEvent(ns, h->ev_switch, 0));
+  h->st = ns;
   return 0;
 }
 
@@ -452,7 +483,9 @@ int sr_upload_scene(sr_handle* h, const float* tris_v, const float* tris_n, cons
   h->n_slots = (int)slots.size();
   h->h_light_area = larea;
   h->h_light_count = lcount;
+  h->h_se = surface_emission_flags(mat_i32, n_mats, B);
   set_camera_host(h, cam);
+  h->gen++;
   h->has_scene = true;
   return 0;
 }
@@ -460,6 +493,7 @@ int sr_upload_scene(sr_handle* h, const float* tris_v, const float* tris_n, cons
 int sr_set_camera(sr_handle* h, const float* cam) {
   if (!h || !cam) return fail(SR_EINVAL, "null argument");
   set_camera_host(h, cam);
+  h->gen++;
   return 0;
 }
 
@@ -469,9 +503,18 @@ int sr_update_materials(sr_handle* h, const float* mat_f32, const int* mat_i32, 
   if (!h) return fail(SR_EINVAL, "null handle");
   if (!h->has_scene) return fail(SR_ESTATE, "no scene uploaded");
   if (n_mats != h->n_mats) return fail(SR_EINVAL, "material count changed (%d -> %d)", h->n_mats, n_mats);
+  if (!mat_i32) return fail(SR_EINVAL, "material arrays must not be NULL");
+  // the light lists, their CDFs, the emitting areas and the light tree were
+  // built from surface_emission: a change there needs a new sr_upload_scene
+  const std::vector<unsigned char> se = surface_emission_flags(mat_i32, n_mats, h->B);
+  for (size_t i = 0; i < se.size(); ++i)
+    if (se[i] != h->h_se[i])
+      return fail(SR_EINVAL, "surface_emission of material %d, band %d changed: re-upload the scene",
+                  (int)(i / (size_t)h->B), (int)(i % (size_t)h->B));
   HIPCHK(hipSetDevice(h->device));
-  // light lists depend on surface_emission: refuse a change there
-  return upload_materials(h, mat_f32, mat_i32, mat_eem, n_mats, eem, n_eem);
+  int rc = upload_materials(h, mat_f32, mat_i32, mat_eem, n_mats, eem, n_eem);
+  if (!rc) h->gen++;
+  return rc;
 }
 
 // Device scene compile: the same records as sr_upload_scene (main.py:509-631),
@@ -536,7 +579,9 @@ int sr_upload_scene_gpu(sr_handle* h, const float* dev_tris_v, const float* dev_
   h->n_slots = o.n_slots;
   h->h_light_area = o.h_light_area;
   h->h_light_count = o.h_light_count;
+  h->h_se = surface_emission_flags(mat_i32, n_mats, h->B);
   set_camera_host(h, cam);
+  h->gen++;
   h->has_scene = true;
   return 0;
 }
@@ -665,6 +710,7 @@ static int ensure_sum(sr_handle* h, int width, int height, hipStream_t st) {
   if (h->sum.bytes < n * 4) {
     int rc = h->sum.ensure(n * 4);
     if (rc) return rc;
+    h->gen++;
     if (sr_launch_fill((float*)h->sum.p, n, 0.0f, st)) return fail(SR_EHIP, "fill launch failed");
   }
   return 0;
@@ -714,6 +760,7 @@ int sr_render(sr_handle* h, int width, int height, uint64_t seed, uint32_t strea
   int grid = sss ? h->sss_grid : h->grid;
   const unsigned long long waves_needed = ((unsigned long long)h->job_host.n_items + 255) / 256;
   if ((unsigned long long)grid > waves_needed) grid = (int)waves_needed;
+  if (queue_overflows(h->job_host.n_items, grid, 256, a.chunk)) return fail(SR_EINVAL, "work queue would wrap");
   HIPCHK(hipEventRecord(h->ev0, st));
   if ((sss ? sr_launch_path_sss(a, grid, 256, st) : sr_launch_path(a, grid, 256, st)))
     return fail(SR_EHIP, "path kernel launch failed");
@@ -763,6 +810,7 @@ int sr_render_adjoint(sr_handle* h, int width, int height, uint64_t seed, uint32
   int grid = h->adj_grid;
   const unsigned long long waves_needed = ((unsigned long long)h->job_host.n_items + 255) / 256;
   if ((unsigned long long)grid > waves_needed) grid = (int)waves_needed;
+  if (queue_overflows(h->job_host.n_items, grid, 256, a.chunk)) return fail(SR_EINVAL, "work queue would wrap");
   HIPCHK(hipEventRecord(h->ev0, st));
   if (sr_launch_adjoint(a, grid, 256, st)) return fail(SR_EHIP, "adjoint kernel launch failed");
   HIPCHK(hipEventRecord(h->ev1, st));
@@ -918,7 +966,7 @@ int sr_batch_submit(sr_batch* b, sr_handle* h, int width, int height, uint64_t s
   if (h->device != b->device) return fail(SR_EINVAL, "scene handle is on another device");
   for (const auto& sp : b->specs)
     if (sp.h == h) return fail(SR_EINVAL, "a scene handle may appear once per batch (its block-sum buffer)");
-  b->specs.push_back({h, width, height, first_block, n_blocks, max_depth, seed, stream, dev_accum, flags});
+  b->specs.push_back({h, width, height, first_block, n_blocks, max_depth, seed, stream, dev_accum, flags, 0});
   b->dirty = true;
   return 0;
 }
@@ -984,14 +1032,19 @@ int sr_batch_launch(sr_batch* b, int flags) {
   if (b->specs.empty()) return fail(SR_ESTATE, "empty batch");
   HIPCHK(hipSetDevice(b->device));
   const int n = (int)b->specs.size();
+  // a job table (and a captured graph) holds each scene's device pointers,
+  // camera and block-sum buffer: re-fill it when any of them changed
+  for (const auto& sp : b->specs)
+    if (sp.gen != sp.h->gen) b->dirty = true;
   if (b->dirty) {
     HIPCHK(hipStreamSynchronize(b->st));
     batch_drop_graph(b);
     b->host_jobs.resize(n);
     for (int j = 0; j < n; ++j) {
-      const auto& sp = b->specs[j];
+      auto& sp = b->specs[j];
       int rc = ensure_sum(sp.h, sp.W, sp.H, b->st);
       if (rc) return rc;
+      sp.gen = sp.h->gen;
       fill_job(sp.h, b->host_jobs[j], sp.W, sp.H, sp.seed, sp.stream, sp.first, sp.nb, sp.depth);
       b->host_jobs[j].nee_tree = ((sp.flags & SR_RENDER_LIGHT_TREE) && sp.h->n_lnodes > 0) ? 1 : 0;
       b->host_jobs[j].sss = (sp.flags & SR_RENDER_SUBSURFACE) ? 1 : 0;
@@ -1006,6 +1059,13 @@ int sr_batch_launch(sr_batch* b, int flags) {
   if ((batch_walks(b) ? sr_path_sss_grid_bs(b->block, &per) : sr_path_grid_bs(b->block, &per)) != 0)
     return fail(SR_EHIP, "occupancy query failed");
   int grid = std::max(1, per * b->grid_pct / 100);
+  {
+    uint32_t max_items = 0;
+    for (const auto& j : b->host_jobs) max_items = std::max(max_items, j.n_items);
+    if (queue_overflows(max_items, grid, b->block, b->chunk))
+      return fail(SR_EINVAL, "queue counter would wrap: %u items + %d waves x chunk %u >= 2^32", max_items,
+                  grid * b->block / 64, b->chunk);
+  }
   HIPCHK(hipEventRecord(b->ev0, b->st));
   if (flags & SR_BATCH_GRAPH) {
     if (!b->exec) {

@@ -70,6 +70,7 @@ class DifferentiableScene:
         self.alb_off = np.full((flat.n_mats,), -1, np.int32)
         for m in spec.albedo_mats:
             self.alb_off[m] = self.offsets[("albedo", m)]
+        self.current_params = None  # the parameters last uploaded by set_params
 
     def initial_params(self):
         """Parameter vector read back from the current material tables."""
@@ -102,6 +103,7 @@ class DifferentiableScene:
             sa = th[self.offsets["sigma_a"]:][:self.B]
             f[self.spec.sigma_mat, PROP_K] = sa * self.lam_m / (4 * math.pi)
         self.r.update_materials(f.astype(np.float32))
+        self.current_params = th.copy()
 
     def render(self, stream=0):
         """Fresh render of `spp` blocks; returns the spectral mean [W, H, B]."""
@@ -126,25 +128,39 @@ class DifferentiableScene:
 
 
 class RenderFunction(torch.autograd.Function):
-    """accum = render(theta); backward runs the path-replay adjoint."""
+    """accum = render(theta) on Philox stream `stream`; backward runs the
+    path-replay adjoint of that same render (same parameters, seed and
+    stream)."""
 
     @staticmethod
-    def forward(ctx, theta, scene: DifferentiableScene):
-        scene.set_params(theta.detach().cpu().numpy())
-        out = scene.render().clone()
+    def forward(ctx, theta, scene: DifferentiableScene, stream=0):
+        th = theta.detach().cpu().numpy().copy()
+        scene.set_params(th)
+        out = scene.render(stream=stream).clone()
         ctx.scene = scene
+        ctx.theta = th  # the adjoint replays the paths of THESE parameters
+        ctx.stream = int(stream)
+        ctx.seed = scene.seed
         ctx.theta_device = theta.device
         ctx.theta_dtype = theta.dtype
         return out
 
     @staticmethod
     def backward(ctx, grad_out):
-        g = ctx.scene.adjoint(grad_out)
-        return g.to(device=ctx.theta_device, dtype=ctx.theta_dtype), None
+        scene = ctx.scene
+        if scene.current_params is None or not np.array_equal(scene.current_params, ctx.theta):
+            scene.set_params(ctx.theta)  # another forward ran on this scene since
+        seed = scene.seed
+        scene.seed = ctx.seed
+        try:
+            g = scene.adjoint(grad_out, stream=ctx.stream)
+        finally:
+            scene.seed = seed
+        return g.to(device=ctx.theta_device, dtype=ctx.theta_dtype), None, None
 
 
-def render(theta, scene: DifferentiableScene):
-    return RenderFunction.apply(theta, scene)
+def render(theta, scene: DifferentiableScene, stream=0):
+    return RenderFunction.apply(theta, scene, int(stream))
 
 
 def inverse_volume_spec():

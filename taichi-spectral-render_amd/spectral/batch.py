@@ -59,10 +59,19 @@ class Batch:
         if renderer.accum is None:
             raise RuntimeError("renderer has no resolution")
         flags = (1 if light_tree else 0) | (2 if subsurface else 0)
+        self._submit(renderer, int(n_blocks), int(max_depth), int(seed), int(stream), flags)
+
+    def _submit(self, renderer, n_blocks, max_depth, seed, stream, flags):
         _lib.call("sr_batch_submit", self._b, renderer._h, renderer.width, renderer.height, ctypes.c_uint64(seed),
-                  ctypes.c_uint32(stream), renderer.blocks_done, int(n_blocks), int(max_depth), flags,
+                  ctypes.c_uint32(stream), renderer.blocks_done, n_blocks, max_depth, flags,
                   _lib.ptr(renderer.accum))
-        self.jobs.append((renderer, int(n_blocks), int(max_depth), int(seed), int(stream)))
+        self.jobs.append((renderer, n_blocks, max_depth, seed, stream, flags, self._state(renderer)))
+
+    @staticmethod
+    def _state(r):
+        """What a submitted job captured of its renderer: where the blocks
+        continue, and the accumulator they fold into."""
+        return (r.blocks_done, r.accum.data_ptr(), r.width, r.height)
 
     def clear(self):
         _lib.call("sr_batch_clear", self._b)
@@ -88,6 +97,16 @@ class Batch:
         return {"total_ms": total.value, "latency_ms": lat, "segments": seg, "traversals": trav}
 
     def run(self, graph=True):
+        """Launch, wait and advance each renderer's block counter.  A job
+        continues from its renderer's blocks_done into its current
+        accumulator: when either moved since the jobs were submitted (a
+        previous run, clear_pixel_space, set_resolution) the jobs are
+        submitted again first (and a graph is captured again)."""
+        if any(self._state(r) != st for (r, *_x, st) in self.jobs):
+            jobs = self.jobs
+            self.clear()
+            for (r, nb, depth, seed, stream, flags, _st) in jobs:
+                self._submit(r, nb, depth, seed, stream, flags)
         self.launch(graph)
         out = self.wait()
         for (r, nb, *_rest) in self.jobs:

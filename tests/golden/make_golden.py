@@ -20,6 +20,18 @@
                    of the flattened scene so a change of the scene generator
                    is caught instead of compared against stale numbers.
 
+  c1_rays.npz      the 1,048,588-face heightfield scene (heightfield_1m): the
+                   closest hits (t, face id, u, v; brute force over every
+                   face) of its 256x256 pixel-centre camera rays (regenerated
+                   by the tests from the scene; their digest is stored) and
+                   of 65,536 seeded rays stored here: 32,768 with uniform
+                   origins and directions in the box and 32,768 aimed from
+                   above at the heightfield;
+  c1_image.npz     the heightfield scene at 32x32, 4096 spectral blocks,
+                   depth 12, seed 1, as c0_image.npz.
+
+    python tests/golden/make_golden.py c1      # only the heightfield fixtures
+
 The oracle is test infrastructure; this script only runs it to freeze its
 outputs.  Every fixture is data (inputs and expected outputs)."""
 import hashlib
@@ -120,7 +132,78 @@ def image():
                         seed=np.int64(cfg.seed), depth=np.int32(cfg.depth), scene_sha256=scene_digest(flat))
 
 
+def c1_flat():
+    cfg = configs.heightfield_1m()
+    return cfg, flatten_scene(cfg.scene, cfg.bands, cfg.wl_min, cfg.wl_max)
+
+
+C1_PIXEL_GRID = 256
+
+
+def c1_random_rays(n, seed):
+    """Half uniform in the box, half aimed from above at the heightfield
+    (its cells span x in [-0.5, 0.5], z in [-0.5, 0.5], heights near -0.45)."""
+    rng = np.random.default_rng(seed)
+    h = n // 2
+    o1 = rng.uniform(-0.49, 0.49, (h, 3))
+    d1 = rng.normal(size=(h, 3))
+    o2 = np.stack([rng.uniform(-0.45, 0.45, n - h), rng.uniform(-0.3, 0.45, n - h),
+                   rng.uniform(-0.45, 0.45, n - h)], 1)
+    tgt = np.stack([rng.uniform(-0.5, 0.5, n - h), np.full(n - h, -0.45), rng.uniform(-0.5, 0.5, n - h)], 1)
+    d2 = tgt - o2
+    d = np.concatenate([d1, d2])
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return np.concatenate([o1, o2]).astype(np.float32), d.astype(np.float32)
+
+
+def ray_digest(o, d):
+    h = hashlib.sha256()
+    h.update(np.ascontiguousarray(o, np.float32).tobytes())
+    h.update(np.ascontiguousarray(d, np.float32).tobytes())
+    return h.hexdigest()
+
+
+def c1_rays():
+    _, flat = c1_flat()
+    po, pd = pixel_center_rays(flat, C1_PIXEL_GRID, C1_PIXEL_GRID)
+    ro, rd = c1_random_rays(65536, 11)
+    o = np.concatenate([po, ro]).astype(np.float32)
+    d = np.concatenate([pd, rd]).astype(np.float32)
+    ora = OracleScene(flat, brute=True)
+    ts, ps, uvs = [], [], []
+    step = 8192
+    for k in range(0, o.shape[0], step):
+        t, p, uv = ora.trace(o[k:k + step], d[k:k + step])
+        ts.append(t), ps.append(p), uvs.append(uv)
+        print(f"c1 rays {k + step}/{o.shape[0]}", flush=True)
+    np.savez_compressed(os.path.join(HERE, "c1_rays.npz"), orig=ro, dirs=rd, t=np.concatenate(ts),
+                        prim=np.concatenate(ps), uv=np.concatenate(uvs), pixel_grid=np.int32(C1_PIXEL_GRID),
+                        pixel_rays_sha256=ray_digest(po, pd), scene_sha256=scene_digest(flat))
+
+
+def c1_image():
+    cfg, flat = c1_flat()
+    ora = OracleScene(flat)
+    chunks = []
+    for c in range(IMG_CHUNKS):
+        acc = np.zeros((IMG_W, IMG_H, cfg.bands), np.float32)
+        a, _ = ora.render(IMG_W, IMG_H, cfg.seed, c * IMG_PER, IMG_PER, cfg.depth, accum=acc)
+        chunks.append(a.astype(np.float64) * (c + 1))
+        print(f"c1 image chunk {c + 1}/{IMG_CHUNKS}", flush=True)
+    chunks = np.stack(chunks)
+    np.savez_compressed(os.path.join(HERE, "c1_image.npz"), mean=chunks.mean(0).astype(np.float32),
+                        sem=(chunks.std(0, ddof=1) / np.sqrt(IMG_CHUNKS)).astype(np.float32),
+                        chunks=np.int32(IMG_CHUNKS), per_chunk=np.int32(IMG_PER), seed=np.int64(cfg.seed),
+                        depth=np.int32(cfg.depth), scene_sha256=scene_digest(flat))
+
+
 if __name__ == "__main__":
-    bsdf_tables()
-    rays()
-    image()
+    if sys.argv[1:] == ["c1"]:
+        c1_image()
+        c1_rays()
+    else:
+        bsdf_tables()
+        rays()
+        image()
+        c1_image()
+        c1_rays()

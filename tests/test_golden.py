@@ -104,3 +104,87 @@ def test_c0_image_independent_stream(box):
     zs = np.sort(z)
     k = len(zs) // 100
     assert abs(zs[k:-k].mean()) < 0.1
+
+
+# ---------------------------------------------------------------- heightfield_1m (1,048,588 faces)
+
+@pytest.fixture(scope="module")
+def hf():
+    cfg = configs.heightfield_1m()
+    return cfg, flatten_scene(cfg.scene, cfg.bands, cfg.wl_min, cfg.wl_max)
+
+
+def c1_all_rays(flat, g):
+    from spectral.scene import pixel_center_rays
+
+    n = int(g["pixel_grid"])
+    po, pd = pixel_center_rays(flat, n, n)
+    return np.concatenate([po, g["orig"]]).astype(np.float32), np.concatenate([pd, g["dirs"]]).astype(np.float32)
+
+
+def test_c1_fixtures_are_current(hf):
+    import hashlib
+
+    _, flat = hf
+    g = _load("c1_rays.npz")
+    for name in ("c1_rays.npz", "c1_image.npz"):
+        assert str(_load(name)["scene_sha256"]) == _digest(flat), f"{name} was made from another scene"
+    o, d = c1_all_rays(flat, g)
+    n = int(g["pixel_grid"]) ** 2
+    h = hashlib.sha256()
+    h.update(o[:n].tobytes())
+    h.update(d[:n].tobytes())
+    assert h.hexdigest() == str(g["pixel_rays_sha256"])
+    assert g["t"].shape == (o.shape[0],)
+
+
+def test_c1_rays_brute_force_sample(hf):
+    """Brute force over all 1,048,588 faces on a sample of the batch (the
+    whole batch took minutes to freeze): bit-exact."""
+    _, flat = hf
+    g = _load("c1_rays.npz")
+    o, d = c1_all_rays(flat, g)
+    idx = np.random.default_rng(5).choice(o.shape[0], 384, replace=False)
+    t, p, uv = OracleScene(flat, brute=True).trace(o[idx], d[idx])
+    assert np.array_equal(p, g["prim"][idx])
+    assert np.array_equal(t, g["t"][idx])
+    hit = p >= 0
+    assert np.array_equal(uv[hit], g["uv"][idx][hit])
+
+
+def test_c1_rays_reference_shaped_bvh(hf):
+    """The oracle's reference-shaped BVH on the whole batch: the same t
+    everywhere, the same face except on exact ties in t; the batch hits the
+    heightfield (material 5) a lot."""
+    _, flat = hf
+    g = _load("c1_rays.npz")
+    o, d = c1_all_rays(flat, g)
+    t, p, _ = OracleScene(flat).trace(o, d)
+    assert np.array_equal(t, g["t"])
+    # a ray through a shared edge of two grid triangles hits both at the
+    # same t: traversal order then picks the face (brute force: the lowest id)
+    assert (p == g["prim"]).mean() > 0.995
+    assert np.array_equal(p >= 0, g["prim"] >= 0)
+    hit = g["prim"] >= 0
+    assert hit.mean() > 0.7
+    assert (flat.tri_mat[g["prim"][hit]] == 5).mean() > 0.4
+
+
+def test_c1_image_independent_stream(hf):
+    """16 blocks from an independent Philox stream against the frozen
+    4096-block heightfield image (robust z statistics, as for the box)."""
+    cfg, flat = hf
+    g = _load("c1_image.npz")
+    W, H = g["mean"].shape[:2]
+    n = 16
+    a, _ = OracleScene(flat).render(W, H, int(g["seed"]), 0, n, int(g["depth"]), stream=1)
+    sem = g["sem"].astype(np.float64)
+    s = np.sqrt(sem ** 2 + sem ** 2 * (int(g["chunks"]) * int(g["per_chunk"]) / n))
+    m = s > 0
+    assert np.array_equal(a[~m], g["mean"][~m])
+    z = (a.astype(np.float64) - g["mean"])[m] / s[m]
+    q50 = np.quantile(np.abs(z), 0.5)
+    assert 0.4 < q50 < 0.85, q50
+    assert (np.abs(z) > 5).mean() < 0.02
+    lit = g["mean"] > 1e-3
+    assert abs(a[lit].sum() / g["mean"][lit].sum() - 1.0) < 0.03

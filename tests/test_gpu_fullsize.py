@@ -18,6 +18,8 @@ from spectral.scene import flatten_scene
 pytestmark = pytest.mark.gpu
 
 COLS = [(300, 302), (700, 702)]
+AGREE = 0.999  # share of pixel-bands that must agree to 1e-3 (relative); measured 0.9997-0.9998
+SUM_REL = 2e-4  # bound on the relative difference of the column sums; measured 0.8e-5-2.3e-5
 
 
 @pytest.fixture(scope="module")
@@ -49,5 +51,7 @@ def test_columns_match_oracle_path_by_path(rendered):
     for i0, i1 in COLS:
         gs, cs = g[i0:i1].astype(np.float64), c[i0:i1].astype(np.float64)
         close = np.abs(gs - cs) <= 1e-3 * np.abs(cs) + 1e-6
-        assert close.mean() >= 0.95, f"columns {i0}:{i1}: {close.mean():.4f} of pixel-bands agree"
-        assert gs.sum() == pytest.approx(cs.sum(), rel=0.02)
+        rel = abs(gs.sum() / cs.sum() - 1.0)
+        print(f"columns {i0}:{i1}: {close.mean():.5f} of pixel-bands agree to 1e-3, column sums differ by {rel:.2e}")
+        assert close.mean() >= AGREE, f"columns {i0}:{i1}: {close.mean():.4f} of pixel-bands agree"
+        assert rel <= SUM_REL
