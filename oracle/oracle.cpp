@@ -618,9 +618,20 @@ struct Counters {
 };
 static inline bool shadow_dense(float sigma_t) { return sigma_t * EPSILON >= 14.5f; }
 
+// OR_PLANT (0 = none) builds the oracle with one deliberate mistake, so
+// that a test can show that its checks fail on it (tests/test_oracle_bsdf.py
+// builds the variants with `make plants`).  Never set in the checker itself.
+#ifndef OR_PLANT
+#define OR_PLANT 0
+#endif
+
 // Oren-Nayar factor shared by scatter_surface and evaluate_bsdf
 static inline float oren_nayar(float roughness, float NdotL, float NdotV, V3 l, V3 v, V3 n) {
+#if OR_PLANT == 6
+  float sigma = roughness;  // planted: sigma without pi / 2
+#else
   float sigma = roughness * PI_F / 2;
+#endif
   float sigma2 = sigma * sigma;
   float A = 1.0f - sigma2 / (2.0f * (sigma2 + 0.33f));
   float B = 0.45f * sigma2 / (sigma2 + 0.09f);
@@ -634,11 +645,16 @@ static inline float oren_nayar(float roughness, float NdotL, float NdotV, V3 l, 
     V3 po = normalize(v - n * NdotV);
     cos_phi_diff = tmax(0.0f, dot(pi, po));
   }
+#if OR_PLANT == 1
+  return A + B * cos_phi_diff * std::tan(alpha) * std::sin(beta);  // planted: sin / tan swapped
+#else
   return A + B * cos_phi_diff * std::sin(alpha) * std::tan(beta);
+#endif
 }
 
 // scatter_surface, main.py:1000-1188
-static SurfaceEvent scatter_surface(const Scene& s, const Ray& ray, const VolumeEvent& ev, Rng& rng) {
+static SurfaceEvent scatter_surface(const Scene& s, const Ray& ray, const VolumeEvent& ev, Rng& rng,
+                                    V3* h_out = nullptr) {
   SurfaceEvent res;
   const int a = ray.active;
   Mat sm = get_mat(s, ev.hit_mat_id, a);
@@ -650,6 +666,7 @@ static SurfaceEvent scatter_surface(const Scene& s, const Ray& ray, const Volume
   float x0 = rng.next();
   float x1 = rng.next();
   V3 H = importance_sample_ggx(x0, x1, N, sm.r);
+  if (h_out) *h_out = H;  // (probe: the sampled half vector)
   float ia = tmax(1e-4f, dot(V, H));
   float F = fresnel_spectral(ia, ni, ki, nt, kt);
   float rnd = clampf(rng.next(), 0.0f, 0.99f);
@@ -664,7 +681,11 @@ static SurfaceEvent scatter_surface(const Scene& s, const Ray& ray, const Volume
       float D = ggx_D(NdotH, sm.r);
       float G = ggx_G(N, V, R, sm.r);
       float bsdf = (D * G * F) / (4.0f * NdotL * NdotV);
+#if OR_PLANT == 2
+      float pdf = (D * NdotH) / ia;  // planted: the 4 of the Jacobian dropped
+#else
       float pdf = (D * NdotH) / (4.0f * ia);
+#endif
       res.direction = R; res.did_reflect = 1;
       float cos_theta = tmax(0.0f, dot(N, R));
       res.estimator = (bsdf * cos_theta) / pdf;
@@ -746,7 +767,11 @@ static SurfaceEvent scatter_surface(const Scene& s, const Ray& ray, const Volume
           float t1 = std::fabs(LdotH * VdotH) / (NdotL * NdotV);
           float t2 = (nt * nt * (1.0f - F) * D * G) / dsq;
           float bsdf = t1 * t2;
+#if OR_PLANT == 3
+          float pdf = (D * NdotH * (1.0f - F) * nt * nt) / dsq;  // planted: |L.H| dropped
+#else
           float pdf = (D * NdotH * (1.0f - F) * nt * nt * std::fabs(LdotH)) / dsq;
+#endif
           res.estimator = (bsdf * NdotL) / pdf;
           res.pdf = pdf;
         }
@@ -850,8 +875,16 @@ static void evaluate_bsdf(const Scene& s, int vol_id, int surf_id, V3 V, V3 L, V
       float NdotH = tmax(1e-4f, dot(N, H));
       float D = ggx_D(NdotH, sm.r);
       float G = ggx_G(N, V, L, sm.r);
+#if OR_PLANT == 5
+      bsdf += (D * G * F) / (4.0f * NdotH * NdotV);  // planted: N.H for N.L
+#else
       bsdf += (D * G * F) / (4.0f * NdotL * NdotV);
+#endif
+#if OR_PLANT == 4
+      pdf += (D * NdotH * F) / VdotH;  // planted: the 4 dropped
+#else
       pdf += (D * NdotH * F) / (4.0f * VdotH);
+#endif
     }
     float absorb = (4.0f * PI_F * k_t * 1e9f) / (s.wl_min + (float)b * s.wl_step);
     float albedo = sm.ss / (sm.ss + absorb + 1e-6f);
@@ -1329,6 +1362,82 @@ float or_ggx_G(const float* n, const float* v, const float* l, float r) {
   return ggx_G(v3(n[0], n[1], n[2]), v3(v[0], v[1], v[2]), v3(l[0], l[1], l[2]), r);
 }
 float or_hg_eval(float c, float g) { return evaluate_henyey_greenstein(c, g); }
+
+// ---- BSDF probes on a one-band, two-material scene: material 0 is the
+// medium the ray travels in (n_i, k_i), material 1 the surface it hits.
+// mat = {n_i, k_i, n_t, k_t, roughness, scattering_coefficient,
+// is_true_volume}; wl_nm is the band's wavelength.
+static Scene probe_scene(const float* mat, float wl_nm) {
+  Scene s;
+  s.B = 1;
+  s.wl_min = wl_nm;
+  s.wl_max = wl_nm + 1.0f;
+  s.wl_step = 1.0f;
+  s.M = 2;
+  s.mat_f32.assign(12, 0.0f);
+  s.mat_i32.assign(4, 0);
+  s.mat_eem.assign(2, 0);
+  s.mat_f32[0] = mat[0];   // n_i
+  s.mat_f32[1] = mat[1];   // k_i
+  s.mat_i32[0] = 1;        // the medium is a volume
+  s.mat_f32[6] = mat[2];   // n_t
+  s.mat_f32[7] = mat[3];   // k_t
+  s.mat_f32[8] = mat[4];   // roughness
+  s.mat_f32[9] = mat[5];   // scattering_coefficient
+  s.mat_i32[2] = (int)mat[6];
+  s.E = 1;
+  s.eem.assign(1, 0.0f);
+  s.nlights.assign(1, 0);
+  s.light_area.assign(1, 0.0f);
+  return s;
+}
+
+float or_oren_nayar(float r, float NdotL, float NdotV, const float* l, const float* v, const float* n) {
+  return oren_nayar(r, NdotL, NdotV, v3(l[0], l[1], l[2]), v3(v[0], v[1], v[2]), v3(n[0], n[1], n[2]));
+}
+
+// n samples of scatter_surface for a ray with direction dir hitting a
+// surface with shading normal nrm; sample i uses Philox counter {i, 0, 0}
+// (slotted blocks of segment 0) with key seed.  out[i] = {direction (3),
+// estimator, pdf, did_reflect, half vector (3)}.
+void or_scatter_surface(const float* mat, float wl_nm, const float* dir, const float* nrm, uint64_t seed, int n,
+                        float* out) {
+  Scene s = probe_scene(mat, wl_nm);
+  for (int i = 0; i < n; ++i) {
+    Ray ray;
+    ray.position = v3(0, 0, 0);
+    ray.direction = v3(dir[0], dir[1], dir[2]);
+    ray.active = ray.transport = 0;
+    ray.vol_mat_id = 0;
+    VolumeEvent ev;
+    ev.hit_mat_id = 1;
+    ev.is_front_face = 1;
+    ev.shading_normal = ev.normal = v3(nrm[0], nrm[1], nrm[2]);
+    Rng rng;
+    rng.key[0] = (uint32_t)seed; rng.key[1] = (uint32_t)(seed >> 32);
+    rng.c0 = (uint32_t)i; rng.c1 = 0; rng.c2 = 0;
+    rng.seg = 0;
+    V3 H;
+    SurfaceEvent se = scatter_surface(s, ray, ev, rng, &H);
+    float* o = out + (size_t)i * 9;
+    o[0] = se.direction.x; o[1] = se.direction.y; o[2] = se.direction.z;
+    o[3] = se.estimator; o[4] = se.pdf; o[5] = (float)se.did_reflect;
+    o[6] = H.x; o[7] = H.y; o[8] = H.z;
+  }
+}
+
+// evaluate_bsdf for n (V, L) pairs about the normal nrm: out[i] = {bsdf, pdf}.
+void or_eval_bsdf(const float* mat, float wl_nm, int n, const float* V, const float* L, const float* nrm, float* out) {
+  Scene s = probe_scene(mat, wl_nm);
+  const V3 N = v3(nrm[0], nrm[1], nrm[2]);
+  for (int i = 0; i < n; ++i) {
+    float b, p;
+    evaluate_bsdf(s, 0, 1, v3(V[3 * i], V[3 * i + 1], V[3 * i + 2]), v3(L[3 * i], L[3 * i + 1], L[3 * i + 2]), N, 0,
+                  b, p);
+    out[2 * i] = b;
+    out[2 * i + 1] = p;
+  }
+}
 // Samplers driven by a path key; out[n*3] directions.
 void or_sample(int which, uint64_t seed, int n, const float* axis, float param, float* out) {
   V3 ax = v3(axis[0], axis[1], axis[2]);

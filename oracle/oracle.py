@@ -5,6 +5,7 @@ the checker and by bench.py's cpu_baseline leg; never by the product.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 from pathlib import Path
 
@@ -13,46 +14,72 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent / "_build" / "liboracle.so"
 _vp = ctypes.c_void_p
 _LIB = None
+_LIBS = {}
 
 
 def load():
     global _LIB
     if _LIB is None:
-        if not LIB_PATH.exists():
-            raise RuntimeError(f"oracle not built: {LIB_PATH} missing (make -C oracle)")
-        lib = ctypes.CDLL(str(LIB_PATH))
-        lib.or_scene_create.restype = _vp
-        lib.or_scene_create.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_float, _vp, _vp, _vp, ctypes.c_int, _vp,
-                                        _vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp, ctypes.c_int]
-        lib.or_scene_destroy.argtypes = [_vp]
-        lib.or_scene_light_tree.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp]
-        lib.or_scene_info.argtypes = [_vp, _vp, _vp]
-        lib.or_trace.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]
-        lib.or_render.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int,
-                                  ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp]
-        lib.or_philox.argtypes = [_vp, _vp, _vp]
-        lib.or_uniforms.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
-                                    _vp]
-        lib.or_fresnel.restype = ctypes.c_float
-        lib.or_fresnel.argtypes = [ctypes.c_float] * 5
-        lib.or_ggx_D.restype = ctypes.c_float
-        lib.or_ggx_D.argtypes = [ctypes.c_float, ctypes.c_float]
-        lib.or_ggx_G.restype = ctypes.c_float
-        lib.or_ggx_G.argtypes = [_vp, _vp, _vp, ctypes.c_float]
-        lib.or_hg_eval.restype = ctypes.c_float
-        lib.or_hg_eval.argtypes = [ctypes.c_float, ctypes.c_float]
-        lib.or_sample.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_int, _vp, ctypes.c_float, _vp]
-        lib.or_shadow.restype = ctypes.c_float
-        lib.or_shadow.argtypes = [_vp, _vp, _vp, ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int]
-        lib.or_nee.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
-                               ctypes.c_int, _vp]
-        lib.or_lt_sample.argtypes = [ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp,
-                                     _vp]
-        lib.or_lt_pdf.argtypes = [ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]
-        lib.or_volume_events.argtypes = [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, _vp,
-                                         _vp, _vp, _vp]
-        _LIB = lib
+        _LIB = _open(LIB_PATH)
     return _LIB
+
+
+@contextlib.contextmanager
+def planted(k):
+    """Every oracle call inside the block goes to the variant built with
+    planted mistake k (OR_PLANT in oracle.cpp; `make -C oracle plants`)."""
+    global _LIB
+    saved = load()
+    _LIB = _open(LIB_PATH.parent / f"liboracle_plant{int(k)}.so")
+    try:
+        yield
+    finally:
+        _LIB = saved
+
+
+def _open(path):
+    key = str(path)
+    if key in _LIBS:
+        return _LIBS[key]
+    if not Path(path).exists():
+        raise RuntimeError(f"oracle not built: {path} missing (make -C oracle)")
+    lib = ctypes.CDLL(key)
+    lib.or_scene_create.restype = _vp
+    lib.or_scene_create.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_float, _vp, _vp, _vp, ctypes.c_int, _vp,
+                                    _vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp, ctypes.c_int]
+    lib.or_scene_destroy.argtypes = [_vp]
+    lib.or_scene_light_tree.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp]
+    lib.or_scene_info.argtypes = [_vp, _vp, _vp]
+    lib.or_trace.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]
+    lib.or_render.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int,
+                              ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp]
+    lib.or_philox.argtypes = [_vp, _vp, _vp]
+    lib.or_uniforms.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
+                                _vp]
+    lib.or_fresnel.restype = ctypes.c_float
+    lib.or_fresnel.argtypes = [ctypes.c_float] * 5
+    lib.or_ggx_D.restype = ctypes.c_float
+    lib.or_ggx_D.argtypes = [ctypes.c_float, ctypes.c_float]
+    lib.or_ggx_G.restype = ctypes.c_float
+    lib.or_ggx_G.argtypes = [_vp, _vp, _vp, ctypes.c_float]
+    lib.or_hg_eval.restype = ctypes.c_float
+    lib.or_hg_eval.argtypes = [ctypes.c_float, ctypes.c_float]
+    lib.or_sample.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_int, _vp, ctypes.c_float, _vp]
+    lib.or_shadow.restype = ctypes.c_float
+    lib.or_shadow.argtypes = [_vp, _vp, _vp, ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.or_nee.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
+                           ctypes.c_int, _vp]
+    lib.or_lt_sample.argtypes = [ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp,
+                                 _vp]
+    lib.or_lt_pdf.argtypes = [ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]
+    lib.or_oren_nayar.restype = ctypes.c_float
+    lib.or_oren_nayar.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp, _vp, _vp]
+    lib.or_scatter_surface.argtypes = [_vp, ctypes.c_float, _vp, _vp, ctypes.c_uint64, ctypes.c_int, _vp]
+    lib.or_eval_bsdf.argtypes = [_vp, ctypes.c_float, ctypes.c_int, _vp, _vp, _vp, _vp]
+    lib.or_volume_events.argtypes = [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, _vp,
+                                     _vp, _vp, _vp]
+    _LIBS[key] = lib
+    return lib
 
 
 def _p(a):
@@ -206,3 +233,31 @@ def sample(which, seed, n, axis, param=0.0):
     out = np.empty((n, 3), np.float32)
     load().or_sample(int(which), int(seed), int(n), _p(_f32(axis)), float(param), _p(out))
     return out
+
+
+def oren_nayar(r, NdotL, NdotV, l, v, n):
+    """The Oren-Nayar factor A + B cos+ sin(alpha) tan(beta) shared by
+    scatter_surface and evaluate_bsdf."""
+    return float(load().or_oren_nayar(float(r), float(NdotL), float(NdotV), _p(_f32(l)), _p(_f32(v)), _p(_f32(n))))
+
+
+def _mat(n_i, k_i, n_t, k_t, r, ss, tv):
+    return np.array([n_i, k_i, n_t, k_t, r, ss, tv], np.float32)
+
+
+def scatter_surface(mat, wl_nm, dirn, nrm, seed, n):
+    """n samples of scatter_surface (probe scene: medium 0 = (n_i, k_i),
+    surface 1 = (n_t, k_t, roughness, sigma_s, is_true_volume)); returns
+    dict of direction [n,3], estimator, pdf, did_reflect, half vector."""
+    out = np.empty((n, 9), np.float32)
+    load().or_scatter_surface(_p(_mat(*mat)), float(wl_nm), _p(_f32(dirn)), _p(_f32(nrm)), int(seed), int(n), _p(out))
+    return {"dir": out[:, 0:3], "est": out[:, 3], "pdf": out[:, 4], "reflect": out[:, 5] > 0.5, "h": out[:, 6:9]}
+
+
+def eval_bsdf(mat, wl_nm, V, L, nrm):
+    """evaluate_bsdf for (V[i], L[i]) about nrm: (bsdf [n], pdf [n])."""
+    V = _f32(V).reshape(-1, 3)
+    L = _f32(L).reshape(-1, 3)
+    out = np.empty((V.shape[0], 2), np.float32)
+    load().or_eval_bsdf(_p(_mat(*mat)), float(wl_nm), V.shape[0], _p(V), _p(L), _p(_f32(nrm)), _p(out))
+    return out[:, 0], out[:, 1]
