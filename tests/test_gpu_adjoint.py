@@ -6,14 +6,17 @@ so finite differences of the forward estimator are the oracle for
 d E[image] / d theta.
 
 Albedo enters only as a factor: central differences with h = 1e-3 on one
-stream are precise, and the adjoint must agree within 5%.  Roughness, g and
-sigma_a also move the sampled directions, free-flight distances and branch
-choices; with common random numbers a perturbation flips a branch in a
-fraction ~h of paths, and each flip contributes O(path radiance / h), so the
-h = 1e-3 difference is dominated by noise.  For those the comparison uses
-several independent streams and a larger step (1e-2 for roughness, 3e-2 for
-g and sigma_a; the curvature bias at these steps is below the tolerance),
-and requires agreement within 5% plus three combined standard errors."""
+stream are precise, and all 93 albedo gradients must agree within 5%.
+Roughness, g and sigma_a also move the sampled directions, free-flight
+distances and branch choices; with common random numbers a perturbation
+flips a branch in a fraction ~h of paths, each flip contributing
+O(path radiance / h), so one stream's difference is noisy.  For those the
+comparison averages 160-224 independent streams (64x64, 256 blocks each) of
+both the adjoint and the difference (h = 1e-2 for roughness, 3e-2 for g, a
+3e-2 relative step of all 31 sigma_a together), and requires the means to
+agree within 5% with no noise allowance: the test first checks that three
+standard errors of the difference of the means are below 2.5% of the
+derivative, so a 5% bound is a bound on the bias."""
 import numpy as np
 import pytest
 import torch
@@ -24,20 +27,37 @@ from spectral.scene import flatten_scene
 
 pytestmark = pytest.mark.gpu
 
-STREAMS = 6
+STREAMS = {"roughness": 160, "g": 224, "sigma_a": 160}  # g's differences are the noisiest
+NOISE_BOUND = 0.025  # 3 standard errors, relative to the derivative
+BIAS_BOUND = 0.05
+
+
+def _scene(gpu, W, spp):
+    cfg = configs.inverse_volume(width=W, height=W, spp=spp, depth=12, seed=3)
+    flat = flatten_scene(cfg.scene, cfg.bands, cfg.wl_min, cfg.wl_max)
+    ds = DifferentiableScene(flat, inverse_volume_spec(), cfg.width, cfg.height, cfg.spp, cfg.depth, cfg.seed,
+                             device=gpu)
+    gen = torch.Generator(device="cpu").manual_seed(7)
+    d_acc = torch.rand((cfg.width, cfg.height, cfg.bands), generator=gen).to(gpu)
+    return ds, d_acc
 
 
 @pytest.fixture(scope="module")
 def setup(gpu):
-    cfg = configs.inverse_volume(width=24, height=24, spp=256, depth=12, seed=3)
-    flat = flatten_scene(cfg.scene, cfg.bands, cfg.wl_min, cfg.wl_max)
-    ds = DifferentiableScene(flat, inverse_volume_spec(), cfg.width, cfg.height, cfg.spp, cfg.depth, cfg.seed,
-                             device=gpu)
+    ds, d_acc = _scene(gpu, 24, 256)
     theta0 = ds.initial_params()
-    gen = torch.Generator(device="cpu").manual_seed(7)
-    d_acc = torch.rand((cfg.width, cfg.height, cfg.bands), generator=gen).to(gpu)
     ds.set_params(theta0)
-    g_adj = np.stack([ds.adjoint(d_acc, stream=s).cpu().numpy().astype(np.float64) for s in range(STREAMS)])
+    g_adj = ds.adjoint(d_acc, stream=0).cpu().numpy().astype(np.float64)[None]
+    return ds, theta0, d_acc, g_adj
+
+
+@pytest.fixture(scope="module")
+def streams(gpu):
+    ds, d_acc = _scene(gpu, 64, 256)
+    theta0 = ds.initial_params()
+    ds.set_params(theta0)
+    n = max(STREAMS.values())
+    g_adj = np.stack([ds.adjoint(d_acc, stream=s).cpu().numpy().astype(np.float64) for s in range(n)])
     return ds, theta0, d_acc, g_adj
 
 
@@ -47,19 +67,9 @@ def _loss(ds, theta, d_acc, stream):
     return float((acc.double() * d_acc.double()).sum())
 
 
-def _fd(ds, theta0, d_acc, i, h, stream):
-    tp = theta0.copy()
-    tm = theta0.copy()
-    tp[i] += h
-    tm[i] -= h
-    return (_loss(ds, tp, d_acc, stream) - _loss(ds, tm, d_acc, stream)) / (2 * h)
-
-
-def _agree(adj, fd):
-    """Means over streams agree within 5% plus 3 combined standard errors."""
-    a, f = adj.mean(0), fd.mean(0)
-    se = np.sqrt(adj.var(0, ddof=1) / len(adj) + fd.var(0, ddof=1) / len(fd))
-    return np.linalg.norm(a - f) <= 0.05 * np.linalg.norm(f) + 3 * np.linalg.norm(se), (a, f, se)
+def _fd(ds, theta0, d_acc, v, h, stream):
+    """Central difference along v with step h (common random numbers)."""
+    return (_loss(ds, theta0 + h * v, d_acc, stream) - _loss(ds, theta0 - h * v, d_acc, stream)) / (2 * h)
 
 
 def test_param_count_and_roundtrip(setup):
@@ -71,27 +81,40 @@ def test_param_count_and_roundtrip(setup):
 
 def test_albedo_gradient_matches_fd_h1e3(setup):
     ds, theta0, d_acc, g_adj = setup
-    idx = list(range(0, 93, 4))
-    fd = np.array([_fd(ds, theta0, d_acc, i, 1e-3, 0) for i in idx])
-    ga = g_adj[0, idx]
+    fd = np.array([_fd(ds, theta0, d_acc, np.eye(126)[i], 1e-3, 0) for i in range(93)])
+    ga = g_adj[0, :93]
     rel = np.linalg.norm(ga - fd) / np.linalg.norm(fd)
-    assert rel < 0.05, (rel, ga[:6], fd[:6])
+    worst = np.max(np.abs(ga - fd) / np.maximum(np.abs(fd), 1e-3 * np.abs(fd).max()))
+    print(f"albedo (93 parameters, h = 1e-3): relative error {rel:.2e}, worst single {worst:.2e}")
+    assert rel < BIAS_BOUND, (rel, ga[:6], fd[:6])
 
 
-@pytest.mark.parametrize("name,i,h", [("roughness", 93, 1e-2), ("g", 94, 3e-2)])
-def test_scalar_gradient_matches_fd(setup, name, i, h):
-    ds, theta0, d_acc, g_adj = setup
-    fd = np.array([[_fd(ds, theta0, d_acc, i, h, s)] for s in range(STREAMS)])
-    ok, info = _agree(g_adj[:, [i]], fd)
-    assert ok, (name, info)
+def _direction(theta0, name):
+    v = np.zeros(126)
+    if name == "roughness":
+        v[93] = 1.0
+    elif name == "g":
+        v[94] = 1.0
+    else:  # all sigma_a together, each scaled by its own value
+        v[95:] = theta0[95:]
+        v /= np.linalg.norm(v)
+    return v
 
 
-def test_sigma_a_gradient_matches_fd(setup):
-    ds, theta0, d_acc, g_adj = setup
-    idx = [i for i in range(95, 126) if theta0[i] > 0.1][::2]
-    fd = np.array([[_fd(ds, theta0, d_acc, i, 3e-2, s) for i in idx] for s in range(STREAMS)])
-    ok, info = _agree(g_adj[:, idx], fd)
-    assert ok, info
+@pytest.mark.parametrize("name,h", [("roughness", 1e-2), ("g", 3e-2), ("sigma_a", 3e-2)])
+def test_gradient_matches_fd_without_noise_allowance(streams, name, h):
+    ds, theta0, d_acc, g_adj = streams
+    v = _direction(theta0, name)
+    n = STREAMS[name]
+    adj = g_adj[:n] @ v
+    fd = np.array([_fd(ds, theta0, d_acc, v, h, s) for s in range(n)])
+    a, f = adj.mean(), fd.mean()
+    se = np.sqrt(adj.var(ddof=1) / n + fd.var(ddof=1) / n)
+    rel, noise = abs(a - f) / abs(f), 3 * se / abs(f)
+    print(f"{name}: adjoint {a:.5g}, finite difference {f:.5g} (h = {h}), relative error {rel:.4f}, "
+          f"3 standard errors {noise:.4f} ({n} streams)")
+    assert noise <= NOISE_BOUND, (name, noise)
+    assert rel <= BIAS_BOUND, (name, rel)
 
 
 def test_autograd_function(setup):
