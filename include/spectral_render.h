@@ -154,6 +154,26 @@ int sr_trace_closest(sr_handle* h, int n, const float* orig, const float* dir, f
 int sr_render(sr_handle* h, int width, int height, uint64_t seed, uint32_t stream, int first_block, int n_blocks,
               int max_depth, int flags, float* dev_accum);
 
+/* sr_render without any device allocation -- no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * sr_render allocates the handle's block-sum buffer on its first call and
+ * whenever the resolution grows; this entry point never calls hipMalloc,
+ * hipFree or any allocating routine.  Its block sums, job table, work
+ * counters and overflow flag live in one fixed device array per GPU
+ * (SR_NOALLOC_MAX_ELEMS floats of block sums, 16 MiB, plus a few hundred
+ * bytes, resident while the library is loaded).  Same arguments, kernels
+ * and results as sr_render; width * height * bands must not exceed
+ * SR_NOALLOC_MAX_ELEMS (SR_EINVAL otherwise).  The array is shared by every
+ * call on that GPU and keeps the last call's contents (the block sums are
+ * cleared at the start of each call): calls that overlap on one GPU --
+ * from two host threads, or on two streams without ordering -- race.
+ * sr_stats reports this call's counters until another sr_render_noalloc
+ * call on the GPU overwrites them. */
+#define SR_NOALLOC_MAX_ELEMS (1u << 22)
+int sr_render_noalloc(sr_handle* h, int width, int height, uint64_t seed, uint32_t stream, int first_block,
+                      int n_blocks, int max_depth, int flags, float* dev_accum);
+
 /* Path-replay adjoint of sr_render (no reference counterpart: the reference
  * has no autodiff).  Replays blocks [first_block, first_block + n_blocks)
  * with the same seed/stream as the forward call and ADDS to dev_grad

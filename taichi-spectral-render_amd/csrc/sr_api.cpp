@@ -6,6 +6,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -99,6 +100,10 @@ struct sr_handle {
   float cam_o[3], cam_u[3], cam_v[3], cam_w[3];
   // work
   DevBuf sum, work, jobs, overflow, tmp, alb_off;
+  // where the last path launch left its counters and overflow flag (the
+  // handle's work area, or the device scratch of sr_render_noalloc)
+  void* last_work = nullptr;
+  uint32_t* last_overflow = nullptr;
   Job job_host;  // staging copy of the one-job table (kept alive for the async copy)
   int grid = 0, adj_grid = 0, sss_grid = 0;
 };
@@ -716,44 +721,53 @@ static int ensure_sum(sr_handle* h, int width, int height, hipStream_t st) {
   return 0;
 }
 
-static int prepare_path_args(sr_handle* h, PathArgs& a, int width, int height, uint64_t seed, uint32_t stream,
-                             int first_block, int n_blocks, int max_depth, int flags = 0) {
+// Device buffers of one path launch: block sums, the one-job table, the
+// work area and the overflow flag.
+struct LaunchBufs {
+  float* sum;
+  void* jobs;
+  void* work;
+  uint32_t* overflow;
+};
+
+static int prepare_path_args(sr_handle* h, PathArgs& a, const LaunchBufs& lb, int width, int height, uint64_t seed,
+                             uint32_t stream, int first_block, int n_blocks, int max_depth, int flags = 0) {
   hipStream_t st = h->st;
-  int rc = ensure_sum(h, width, height, st);
-  if (rc) return rc;
   fill_job(h, h->job_host, width, height, seed, stream, first_block, n_blocks, max_depth);
+  h->job_host.sum = lb.sum;
   h->job_host.nee_tree = ((flags & SR_RENDER_LIGHT_TREE) && h->n_lnodes > 0) ? 1 : 0;
   h->job_host.sss = (flags & SR_RENDER_SUBSURFACE) ? 1 : 0;
-  HIPCHK(hipMemcpyAsync(h->jobs.p, &h->job_host, sizeof(Job), hipMemcpyHostToDevice, st));
-  if (reset_work(h->work.p, 1, st)) return fail(SR_EHIP, "reset launch failed");
-  HIPCHK(hipMemsetAsync(h->overflow.p, 0, 4, st));
+  HIPCHK(hipMemcpyAsync(lb.jobs, &h->job_host, sizeof(Job), hipMemcpyHostToDevice, st));
+  if (reset_work(lb.work, 1, st)) return fail(SR_EHIP, "reset launch failed");
+  if (sr_launch_fill_u32(lb.overflow, 1, 0u, st)) return fail(SR_EHIP, "reset launch failed");
   std::memset(&a, 0, sizeof(a));
-  WorkView w = work_view(h->work.p, 1);
-  a.jobs = (const Job*)h->jobs.p;
+  WorkView w = work_view(lb.work, 1);
+  a.jobs = (const Job*)lb.jobs;
   a.n_jobs = 1;
   a.queues = w.queues;
   a.done_items = w.done;
   a.clocks = w.clocks;
   a.counters = w.counters;
-  a.overflow = (uint32_t*)h->overflow.p;
+  a.overflow = lb.overflow;
   a.chunk = 64;
   a.adj.rough_mat = a.adj.g_mat = a.adj.sig_mat = -2;
   a.adj.rough_idx = a.adj.g_idx = a.adj.sig_off = -1;
+  h->last_work = lb.work;
+  h->last_overflow = lb.overflow;
   return 0;
 }
 
-int sr_render(sr_handle* h, int width, int height, uint64_t seed, uint32_t stream, int first_block, int n_blocks,
-              int max_depth, int flags, float* dev_accum) {
-  g_err.clear();
-  if (flags & ~(SR_RENDER_LIGHT_TREE | SR_RENDER_SUBSURFACE)) return fail(SR_EINVAL, "unknown render flags 0x%x", flags);
-  int rc = check_render_args(h, width, height, first_block, n_blocks, max_depth, stream);
-  if (rc) return rc;
-  if (!dev_accum) return fail(SR_EINVAL, "dev_accum is NULL");
-  HIPCHK(hipSetDevice(h->device));
-  if (n_blocks == 0) return 0;
+static LaunchBufs handle_bufs(sr_handle* h) {
+  return LaunchBufs{(float*)h->sum.p, h->jobs.p, h->work.p, (uint32_t*)h->overflow.p};
+}
+
+// The path launch and the fold of its blocks into dev_accum (sr_render and
+// sr_render_noalloc; the arguments are checked by the caller).
+static int render_with(sr_handle* h, const LaunchBufs& lb, int width, int height, uint64_t seed, uint32_t stream,
+                       int first_block, int n_blocks, int max_depth, int flags, float* dev_accum) {
   PathArgs a;
-  if ((rc = prepare_path_args(h, a, width, height, seed, stream, first_block, n_blocks, max_depth, flags)))
-    return rc;
+  int rc = prepare_path_args(h, a, lb, width, height, seed, stream, first_block, n_blocks, max_depth, flags);
+  if (rc) return rc;
   hipStream_t st = h->st;
   const bool sss = (flags & SR_RENDER_SUBSURFACE) != 0;
   if (sss && h->sss_grid == 0 && sr_path_sss_grid(&h->sss_grid)) return fail(SR_EHIP, "occupancy query failed");
@@ -767,9 +781,79 @@ int sr_render(sr_handle* h, int width, int height, uint64_t seed, uint32_t strea
   HIPCHK(hipEventRecord(h->ev1, st));
   h->timed = true;
   const size_t n = (size_t)width * height * h->B;
-  if (sr_launch_finalize(dev_accum, (float*)h->sum.p, n, first_block, n_blocks, st))
+  if (sr_launch_finalize(dev_accum, lb.sum, n, first_block, n_blocks, st))
     return fail(SR_EHIP, "finalize launch failed");
   return 0;
+}
+
+int sr_render(sr_handle* h, int width, int height, uint64_t seed, uint32_t stream, int first_block, int n_blocks,
+              int max_depth, int flags, float* dev_accum) {
+  g_err.clear();
+  if (flags & ~(SR_RENDER_LIGHT_TREE | SR_RENDER_SUBSURFACE)) return fail(SR_EINVAL, "unknown render flags 0x%x", flags);
+  int rc = check_render_args(h, width, height, first_block, n_blocks, max_depth, stream);
+  if (rc) return rc;
+  if (!dev_accum) return fail(SR_EINVAL, "dev_accum is NULL");
+  HIPCHK(hipSetDevice(h->device));
+  if (n_blocks == 0) return 0;
+  if ((rc = ensure_sum(h, width, height, h->st))) return rc;
+  return render_with(h, handle_bufs(h), width, height, seed, stream, first_block, n_blocks, max_depth, flags,
+                     dev_accum);
+}
+
+// ---------------------------------------------------------------- allocation-free render
+// sr_render_noalloc keeps every buffer of its launch -- the block sums, the
+// one-job table, the work area and the overflow flag -- in this file-scope
+// device array (one copy per GPU, resident while the library is loaded).
+// Host code never uses the array's name as an address: each device's
+// address comes from hipGetSymbolAddress on that device's first call.
+namespace {
+constexpr size_t kScratchSumBytes = (size_t)SR_NOALLOC_MAX_ELEMS * sizeof(float);
+constexpr size_t kScratchJobOff = kScratchSumBytes;
+constexpr size_t kScratchWorkOff = kScratchJobOff + (sizeof(Job) + 255) / 256 * 256;
+constexpr size_t kScratchOvfOff = kScratchWorkOff + 256;
+constexpr size_t kScratchBytes = kScratchOvfOff + 256;
+static_assert(sizeof(Job) % 8 == 0, "the job table is copied whole");
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// The calling thread's current device must be `device`.
+static int scratch_for_device(int device, unsigned char** out) {
+  if (device < 0 || device >= kMaxDevices) return fail(SR_EINVAL, "device %d outside [0, %d)", device, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(SR_EHIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+    g_scratch_ptr[device] = p;
+  }
+  *out = (unsigned char*)g_scratch_ptr[device];
+  return 0;
+}
+
+int sr_render_noalloc(sr_handle* h, int width, int height, uint64_t seed, uint32_t stream, int first_block,
+                      int n_blocks, int max_depth, int flags, float* dev_accum) {
+  g_err.clear();
+  if (flags & ~(SR_RENDER_LIGHT_TREE | SR_RENDER_SUBSURFACE)) return fail(SR_EINVAL, "unknown render flags 0x%x", flags);
+  int rc = check_render_args(h, width, height, first_block, n_blocks, max_depth, stream);
+  if (rc) return rc;
+  if (!dev_accum) return fail(SR_EINVAL, "dev_accum is NULL");
+  const unsigned long long n = (unsigned long long)width * height * (unsigned long long)h->B;
+  if (n > SR_NOALLOC_MAX_ELEMS)
+    return fail(SR_EINVAL, "width * height * bands = %llu exceeds the %u elements sr_render_noalloc holds", n,
+                (unsigned)SR_NOALLOC_MAX_ELEMS);
+  HIPCHK(hipSetDevice(h->device));
+  if (n_blocks == 0) return 0;
+  unsigned char* base = nullptr;
+  if ((rc = scratch_for_device(h->device, &base))) return rc;
+  LaunchBufs lb{(float*)base, base + kScratchJobOff, base + kScratchWorkOff, (uint32_t*)(base + kScratchOvfOff)};
+  // the block sums start at zero (a fill kernel: no memset node if captured)
+  if (sr_launch_fill(lb.sum, (size_t)n, 0.0f, h->st)) return fail(SR_EHIP, "fill launch failed");
+  return render_with(h, lb, width, height, seed, stream, first_block, n_blocks, max_depth, flags, dev_accum);
 }
 
 int sr_render_adjoint(sr_handle* h, int width, int height, uint64_t seed, uint32_t stream, int first_block,
@@ -791,8 +875,10 @@ int sr_render_adjoint(sr_handle* h, int width, int height, uint64_t seed, uint32
   if (sig_mat >= 0 && (sig_off < 0 || sig_off + B > n_params)) return fail(SR_EINVAL, "bad sig_off");
   HIPCHK(hipSetDevice(h->device));
   if (n_blocks == 0) return 0;
+  if ((rc = ensure_sum(h, width, height, h->st))) return rc;
   PathArgs a;
-  if ((rc = prepare_path_args(h, a, width, height, seed, stream, first_block, n_blocks, max_depth))) return rc;
+  if ((rc = prepare_path_args(h, a, handle_bufs(h), width, height, seed, stream, first_block, n_blocks, max_depth)))
+    return rc;
   if ((rc = h->alb_off.upload(alb_off, (size_t)M, h->st))) return rc;
   a.adj.d_accum = dev_d_accum;
   a.adj.grad = dev_grad;
@@ -882,8 +968,10 @@ int sr_stats(sr_handle* h, uint64_t* segments, uint64_t* traversals, uint64_t* n
   HIPCHK(hipSetDevice(h->device));
   unsigned long long c[4] = {0, 0, 0, 0};
   uint32_t ovf = 0;
-  HIPCHK(hipMemcpyAsync(c, work_view(h->work.p, 1).counters, sizeof(c), hipMemcpyDeviceToHost, h->st));
-  HIPCHK(hipMemcpyAsync(&ovf, h->overflow.p, 4, hipMemcpyDeviceToHost, h->st));
+  void* work = h->last_work ? h->last_work : h->work.p;
+  uint32_t* ovp = h->last_overflow ? h->last_overflow : (uint32_t*)h->overflow.p;
+  HIPCHK(hipMemcpyAsync(c, work_view(work, 1).counters, sizeof(c), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipMemcpyAsync(&ovf, ovp, 4, hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
   if (segments) *segments = c[0];
   if (traversals) *traversals = c[1];

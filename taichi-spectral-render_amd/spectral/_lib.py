@@ -43,6 +43,8 @@ SIGNATURES = {
     "sr_trace_closest": ([_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, ctypes.c_int], ctypes.c_int),
     "sr_render": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int, ctypes.c_int,
                    ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
+    "sr_render_noalloc": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int,
+                           ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
     "sr_render_adjoint": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int,
                            ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                            ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
