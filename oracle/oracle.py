@@ -7,11 +7,13 @@ from __future__ import annotations
 
 import contextlib
 import ctypes
+import os
 from pathlib import Path
 
 import numpy as np
 
-LIB_PATH = Path(__file__).resolve().parent / "_build" / "liboracle.so"
+# ORACLE_LIB: another build of the same checker (the sanitizer build of `make asan`)
+LIB_PATH = Path(os.environ.get("ORACLE_LIB") or Path(__file__).resolve().parent / "_build" / "liboracle.so")
 _vp = ctypes.c_void_p
 _LIB = None
 _LIBS = {}
@@ -30,7 +32,7 @@ def planted(k):
     planted mistake k (OR_PLANT in oracle.cpp; `make -C oracle plants`)."""
     global _LIB
     saved = load()
-    _LIB = _open(LIB_PATH.parent / f"liboracle_plant{int(k)}.so")
+    _LIB = _open(Path(__file__).resolve().parent / "_build" / f"liboracle_plant{int(k)}.so")
     try:
         yield
     finally:
