@@ -141,7 +141,10 @@ int sr_trace_closest(sr_handle* h, int n, const float* orig, const float* dir, f
  * the blocks into the running mean dev_accum [W][H][B] exactly as
  * average_and_reset_block (main.py:1592-1607) does once per block.
  * Randomness: Philox-4x32-10 keyed by seed, counter (i*H + j, block,
- * band | stream << 8, call/4).  flags: 0 or SR_RENDER_LIGHT_TREE | SR_RENDER_SUBSURFACE. */
+ * band | stream << 8, call/4).  The work items of a call -- W * H *
+ * n_blocks * ceil(B / 8), each up to 8 bands of one pixel and block -- must
+ * number fewer than 2^31.  flags: 0 or SR_RENDER_LIGHT_TREE |
+ * SR_RENDER_SUBSURFACE. */
 #define SR_RENDER_LIGHT_TREE 1 /* NEE picks its light through the light tree (importance by distance and
                                   orientation; the TODO of main.py:541) instead of the area CDF
                                   (main.py:1306-1313): same expected image, lower variance with many

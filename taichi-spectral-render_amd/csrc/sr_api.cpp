@@ -688,8 +688,11 @@ static int check_render_args(sr_handle* h, int width, int height, int first_bloc
   if (first_block < 0 || n_blocks < 0) return fail(SR_EINVAL, "bad block range");
   if (max_depth < 0) return fail(SR_EINVAL, "max_depth < 0");
   if (stream > 0xFFFFFFu) return fail(SR_EINVAL, "stream id must fit 24 bits");
-  const unsigned long long items = (unsigned long long)width * height * (unsigned long long)n_blocks;
-  if (items >= 0x7FFFFFFFull) return fail(SR_EINVAL, "W*H*n_blocks must be < 2^31 per call");
+  // work items: pixels x blocks x band groups
+  const unsigned long long items =
+      (unsigned long long)width * height * (unsigned long long)n_blocks * sr_items_per_pixel(h->B);
+  if (items >= 0x7FFFFFFFull) return fail(SR_EINVAL, "W*H*n_blocks*ceil(bands/%d) must be < 2^31 per call",
+                                          SR_BANDS_PER_ITEM);
   return 0;
 }
 
@@ -698,7 +701,7 @@ static void fill_job(sr_handle* h, Job& j, int width, int height, uint64_t seed,
   std::memset(&j, 0, sizeof(j));
   fill_scene_dev(h, j.s);
   j.sum = (float*)h->sum.p;
-  j.n_items = (uint32_t)((unsigned long long)width * height * (unsigned long long)n_blocks);
+  j.n_items = (uint32_t)((unsigned long long)width * height * (unsigned long long)n_blocks * sr_items_per_pixel(h->B));
   j.W = width;
   j.H = height;
   j.depth = max_depth;

@@ -10,6 +10,15 @@
 
 #define SR_MAX_PARAMS 1024
 
+// Bands per work item of the path kernel: a lane runs that many single-band
+// paths of one pixel and block back to back (the same pixel's rays stay
+// close to each other), and a block of W x H pixels is W * H * ceil(B / this)
+// items, small enough for the persistent grid to balance at a launch's end.
+#ifndef SR_BANDS_PER_ITEM
+#define SR_BANDS_PER_ITEM 8
+#endif
+static inline uint32_t sr_items_per_pixel(int bands) { return (uint32_t)((bands + SR_BANDS_PER_ITEM - 1) / SR_BANDS_PER_ITEM); }
+
 // Scene records built on the device by sr_build_scene_device (sr_build.hip);
 // every pointer is a hipMalloc allocation the caller takes over (or frees
 // with sr_free_scene_out).

@@ -25,7 +25,9 @@
 #include "sr_dual.h"
 #include "sr_kernels.h"
 
-enum { PH_NEW = 0, PH_EXT = 1, PH_SHD = 2, PH_REPLAY = 3, PH_PARK = 4, PH_FIN = 5 };
+// PH_NEW: a path just ended; PH_NEXT: start the item's next band's path
+enum { PH_NEW = 0, PH_EXT = 1, PH_SHD = 2, PH_REPLAY = 3, PH_PARK = 4, PH_FIN = 5, PH_NEXT = 6 };
+
 
 struct Lane {
   int phase, band, pi, pj, blk;
@@ -981,19 +983,40 @@ __device__ __forceinline__ void on_shadow(Lane& L, const Ctx<ADJ>& c, const HitD
   L.phase = PH_FIN;
 }
 
-__device__ __forceinline__ void item_to_pixel(const Job& a, uint32_t item, int& pi, int& pj, int& blk) {
+// A work item is a run of up to SR_BANDS_PER_ITEM single-band paths of one
+// pixel and block: (pixel, block, band group).  Within a block the items run
+// tile by tile (8x8 pixels) and, in a tile, group by group, so the 64
+// consecutive items a wave claims are one tile's pixels (coherent camera
+// rays); an image whose sides are not multiples of 8 runs group by group
+// over whole columns.  A lane runs its item's bands back to back (the same
+// pixel, so its rays stay close to the last ones), and items are small
+// enough that the persistent grid still balances at the end of a launch.
+__device__ __forceinline__ uint32_t band_groups(const Job& a) {
+  return ((uint32_t)a.s.B + SR_BANDS_PER_ITEM - 1) / SR_BANDS_PER_ITEM;
+}
+__device__ __forceinline__ void item_to_path(const Job& a, uint32_t item, int& pi, int& pj, int& blk, int& band,
+                                             int& band_end) {
   const uint32_t npix = (uint32_t)a.W * (uint32_t)a.H;
-  blk = a.first_block + (int)(item / npix);
-  uint32_t r = item % npix;
+  const uint32_t G = band_groups(a);
+  const uint32_t per_block = npix * G;
+  blk = a.first_block + (int)(item / per_block);
+  const uint32_t r = item % per_block;
+  uint32_t g;
   if (a.tiled) {
-    uint32_t tile = r >> 6, w = r & 63u;
-    uint32_t tx = tile % (uint32_t)(a.W >> 3), ty = tile / (uint32_t)(a.W >> 3);
+    const uint32_t tile = r / (64u * G), rem = r % (64u * G);
+    const uint32_t w = rem & 63u;
+    g = rem >> 6;
+    const uint32_t tx = tile % (uint32_t)(a.W >> 3), ty = tile / (uint32_t)(a.W >> 3);
     pi = (int)(tx * 8 + (w & 7u));
     pj = (int)(ty * 8 + (w >> 3));
   } else {
-    pi = (int)(r / (uint32_t)a.H);
-    pj = (int)(r % (uint32_t)a.H);
+    g = r / npix;
+    const uint32_t p = r % npix;
+    pi = (int)(p / (uint32_t)a.H);
+    pj = (int)(p % (uint32_t)a.H);
   }
+  band = (int)(g * SR_BANDS_PER_ITEM);
+  band_end = min(band + SR_BANDS_PER_ITEM, a.s.B);
 }
 
 // Per-wave flush of the lane counters into job j's slot.
@@ -1034,6 +1057,7 @@ __device__ __forceinline__ void path_loop(const PathArgs& P, int* stk, float* ld
   L.Lw = L.Eb = L.sp_dr = L.stot_dsig = L.hg_sc = 0.0f;
   L.fdirect = 0.0f;
   bool has_item = false;
+  int band_end = 0;  // the lane's item covers bands [.., band_end)
   // The job table is written only by the host: read it through the constant
   // address space so every field access is a (cached) scalar load.
   typedef __attribute__((address_space(4))) const Job ConstJob;
@@ -1042,15 +1066,24 @@ __device__ __forceinline__ void path_loop(const PathArgs& P, int* stk, float* ld
     cur = __builtin_amdgcn_readfirstlane(cur);
     const Job& J = *(const Job*)(jobs + cur);
     const Ctx<ADJ> c{J, P.adj, lds_grad};
-    if (L.phase == PH_NEW) {
+    if (L.phase == PH_NEW) {  // a path ended: the item's next band, if any
       L.band += 1;
       L.pass = 0;
-      if (L.band >= J.s.B) {  // item finished: count it, stamp the job's completion
-        if (has_item) {
-          has_item = false;
-          uint32_t d = atomicAdd(P.done_items + cur, 1u) + 1u;
-          if (d == J.n_items) P.clocks[1 + cur] = wall_clock64();
+      if (L.band < band_end) L.phase = PH_NEXT;
+    }
+    {  // items whose last path ended: one count per wave, the job's completion stamped by the last
+      const bool fin = L.phase == PH_NEW && has_item;
+      const unsigned long long fmask = __ballot(fin);
+      if (fmask) {
+        const int leader = __ffsll((long long)fmask) - 1;
+        if (lane == leader) {
+          const uint32_t k = (uint32_t)__popcll(fmask);
+          if (atomicAdd(P.done_items + cur, k) + k == J.n_items) P.clocks[1 + cur] = wall_clock64();
         }
+      }
+      if (L.phase == PH_NEW) {
+        L.pass = 0;
+        has_item = false;
         L.phase = PH_PARK;
       }
     }
@@ -1075,10 +1108,10 @@ __device__ __forceinline__ void path_loop(const PathArgs& P, int* stk, float* ld
         if (r < left) item = lbase + r;
         else if (nb + (r - left) < nend) item = nb + (r - left);
         if (item != 0xFFFFFFFFu) {
-          item_to_pixel(J, item, L.pi, L.pj, L.blk);
-          L.band = 0;
+          item_to_path(J, item, L.pi, L.pj, L.blk, L.band, band_end);
+          L.pass = 0;
           has_item = true;
-          L.phase = PH_NEW;
+          L.phase = PH_NEXT;
         }
       }
       if (left >= n) {
@@ -1088,7 +1121,7 @@ __device__ __forceinline__ void path_loop(const PathArgs& P, int* stk, float* ld
         lend = nend;
       }
     }
-    if (L.phase == PH_NEW || L.phase == PH_REPLAY) start_path<ADJ>(L, c);
+    if (L.phase == PH_NEXT || L.phase == PH_REPLAY) start_path<ADJ>(L, c);
     if (__ballot(L.phase != PH_PARK) == 0ull) {
       // every lane is parked: the job's queue is dry for this wave and none
       // of its lanes still runs a path of it -- move the wave to the next job

@@ -215,7 +215,8 @@ def test_upload_rejects_bad_material_ids(gpu):
 
 def test_render_argument_limits(gpu):
     """sr_render rejects what its kernels cannot index, before any launch:
-    more than 2^31 - 1 work items (W x H x n_blocks) in one call, negative
+    more than 2^31 - 1 work items (W x H x n_blocks x ceil(bands / 8): a
+    work item is up to 8 bands of one pixel and block) in one call, negative
     blocks or depth, stream ids beyond 24 bits (the RNG counter field) and
     a zero resolution; a call without a scene is out of order.  Zero blocks
     is a no-op that leaves the accumulator untouched."""
@@ -230,7 +231,7 @@ def test_render_argument_limits(gpu):
     r = Renderer(flat, cfg.bands, cfg.wl_min, cfg.wl_max, width=8, height=8, device=gpu)
     acc = ptr(r.accum)
     bad = [
-        ((8, 8, 0, 0, 0, (1 << 31) // 64, 4, 0), "2\\^31"),
+        ((8, 8, 0, 0, 0, (1 << 31) // (64 * -(-cfg.bands // 8)) + 1, 4, 0), "2\\^31"),
         ((8, 8, 0, 0, -1, 1, 4, 0), "block range"),
         ((8, 8, 0, 0, 0, -1, 4, 0), "block range"),
         ((8, 8, 0, 0, 0, 1, -1, 0), "max_depth"),
