@@ -153,15 +153,27 @@ __device__ __forceinline__ void intersect_tri(f3 o, f3 d, float4 a, float4 b, fl
 
 // Tests the 4 children of node `cur` against the ray; returns the nearest
 // hit child in `next` (SR_EMPTY when none) and pushes the others far-to-near.
+// The near and far planes of each axis are fetched by address -- `ox`, `oy`,
+// `oz` are the byte offsets of the near plane's float4 in the node line
+// (lo_x at 0, hi_x at 16, lo_y at 32, ...: hi when the ray's direction is
+// negative on that axis; the far plane is the other one, offset ^ 16) -- so
+// no per-child selects are needed.  The child order
+// comes from a branch-free sorting network; the result does not depend on
+// the order (ties in t go to the smaller face id).
 __device__ __forceinline__ void visit_node(const Node4* __restrict__ nodes, int cur, f3 inv, f3 oi, bool nx, bool ny,
                                            bool nz, float tmax_, int* stk, int stride, int& sp, int& next,
                                            uint32_t* overflow) {
-  const float4* np = reinterpret_cast<const float4*>(nodes + cur);
-  float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-  int4 ch = *reinterpret_cast<const int4*>(np + 6);
-  float4 nxv = nx ? hx : lx, fxv = nx ? lx : hx;
-  float4 nyv = ny ? hy : ly, fyv = ny ? ly : hy;
-  float4 nzv = nz ? hz : lz, fzv = nz ? lz : hz;
+  // 32-bit byte offsets from the array base (kept in scalar registers): one
+  // VGPR per address instead of a 64-bit pointer
+  const char* base = reinterpret_cast<const char*>(nodes);
+  const uint32_t nb = (uint32_t)cur * (uint32_t)sizeof(Node4);
+  const uint32_t ox = nb + (nx ? 16u : 0u), oy = nb + (ny ? 48u : 32u), oz = nb + (nz ? 80u : 64u);
+#define SR_LD4(off) (*reinterpret_cast<const float4*>(base + (size_t)(uint32_t)(off)))
+  const float4 nxv = SR_LD4(ox), fxv = SR_LD4(ox ^ 16u);
+  const float4 nyv = SR_LD4(oy), fyv = SR_LD4(oy ^ 16u);
+  const float4 nzv = SR_LD4(oz), fzv = SR_LD4(oz ^ 16u);
+#undef SR_LD4
+  const int4 ch = *reinterpret_cast<const int4*>(base + (size_t)(nb + 96u));
   float tn[4], tf[4];
   const float* pnx = &nxv.x; const float* pfx = &fxv.x;
   const float* pny = &nyv.x; const float* pfy = &fyv.x;
@@ -179,7 +191,7 @@ __device__ __forceinline__ void visit_node(const Node4* __restrict__ nodes, int 
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     // conservative slack for the fma-rounded slab distances
-    bool hit = tn[c] <= tf[c] * 1.0000005f + 1e-7f && cref[c] != SR_EMPTY;
+    bool hit = tn[c] <= fmaf(tf[c], 1.0000005f, 1e-7f) && cref[c] != SR_EMPTY;
     key[c] = hit ? tn[c] : INFINITY;
     hitc += hit ? 1 : 0;
   }
@@ -188,20 +200,27 @@ __device__ __forceinline__ void visit_node(const Node4* __restrict__ nodes, int 
     return;
   }
 #define SR_CSWAP(i, j)                                                   \
-  if (key[j] < key[i]) {                                                 \
-    float tk = key[i]; key[i] = key[j]; key[j] = tk;                     \
-    int tr = cref[i]; cref[i] = cref[j]; cref[j] = tr;                   \
+  {                                                                      \
+    const bool sw = key[j] < key[i];                                     \
+    const float ki = key[i], kj = key[j];                                \
+    const int ri = cref[i], rj = cref[j];                                \
+    key[i] = sw ? kj : ki; key[j] = sw ? ki : kj;                        \
+    cref[i] = sw ? rj : ri; cref[j] = sw ? ri : rj;                      \
   }
   SR_CSWAP(0, 1) SR_CSWAP(2, 3) SR_CSWAP(0, 2) SR_CSWAP(1, 3) SR_CSWAP(1, 2)
 #undef SR_CSWAP
   next = cref[0];
+  // push the other hits far-to-near: cref[c] goes to sp + np - c, so the
+  // nearest of them ends on top (the builder bounds the depth so that the
+  // stack cannot overflow; an overflow is still flagged, never written)
+  const int np = hitc - 1;
+  if (sp + np > SR_STACK_SIZE) atomicOr(overflow, 1u);
 #pragma unroll
-  for (int c = 3; c >= 1; --c) {
-    if (c < hitc) {
-      if (sp < SR_STACK_SIZE) stk[(sp++) * stride] = cref[c];
-      else atomicOr(overflow, 1u);
-    }
+  for (int c = 1; c <= 3; ++c) {
+    const int slot = sp + np - c;
+    if (c <= np && slot < SR_STACK_SIZE) stk[slot * stride] = cref[c];
   }
+  sp = min(sp + np, SR_STACK_SIZE);
 }
 
 // All SR_LEAF_MAX (4) triangle records of a leaf are fetched before any is
@@ -259,6 +278,7 @@ __device__ __forceinline__ HitD trace_closest(const Node4* __restrict__ nodes, c
              fabsf(d.z) < tiny ? copysignf(tiny, d.z) : d.z);
   f3 inv = mk(1.0f / dd.x, 1.0f / dd.y, 1.0f / dd.z);
   f3 oi = mk(o.x * inv.x, o.y * inv.y, o.z * inv.z);
+  // the near plane of an axis is the box's hi side when the direction is negative
   const bool nx = inv.x < 0.f, ny = inv.y < 0.f, nz = inv.z < 0.f;
   int sp = 0;
   int cur = 0;       // root node; SR_EMPTY = nothing left
