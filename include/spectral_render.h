@@ -62,7 +62,9 @@ int sr_create(int bands, float wl_min, float wl_max, sr_handle** out);
 int sr_destroy(sr_handle* h);
 
 /* Orders all work of `h` on `stream` (a hipStream_t; NULL = the handle's own
- * default-priority stream). */
+ * default-priority stream).  Work already queued on the previous stream
+ * completes before work queued on the new one (an event wait, no host
+ * synchronization). */
 int sr_set_stream(sr_handle* h, void* stream);
 
 /* Replaces setup_scene (main.py:509-631) and the field allocation of
@@ -101,7 +103,10 @@ int sr_upload_scene_gpu(sr_handle* h, const float* dev_tris_v, const float* dev_
 /* update_camera (main.py:633-640): new origin/lookat/up for the next render. */
 int sr_set_camera(sr_handle* h, const float* cam);
 
-/* Material table update without rebuilding geometry (same layout as upload). */
+/* Material table update without rebuilding geometry (same layout as upload).
+ * SR_EINVAL if any (material, band) surface_emission flag differs from the
+ * uploaded scene's: the light lists, their CDFs, the emitting areas and the
+ * light tree were built from those flags (upload the scene again). */
 int sr_update_materials(sr_handle* h, const float* mat_f32, const int* mat_i32, const int* mat_eem, int n_mats,
                         const float* eem, int n_eem);
 
@@ -245,7 +250,12 @@ int sr_batch_clear(sr_batch* b);
 /* Launch shape: workgroup size (64/128/256), grid as a percentage of the
  * occupancy-limited resident grid, items claimed per queue atomic. */
 int sr_batch_config(sr_batch* b, int block, int grid_pct, int chunk);
-/* Enqueues the batch on the batch's own stream (asynchronous). */
+/* Enqueues the batch on the batch's own stream (asynchronous): every job's
+ * submitted block range.  A job whose scene handle changed since the last
+ * launch (upload, camera, materials, a larger block-sum buffer) has its
+ * job-table entry re-filled (and a captured graph captured again).  The
+ * queue counter of a job must not wrap: SR_EINVAL when items + (waves + 1) x
+ * chunk reaches 2^32. */
 int sr_batch_launch(sr_batch* b, int flags);
 /* Blocks; total device time, and per job (arrays of length = jobs, may be
  * NULL): completion latency from the launch's first wave (device wall

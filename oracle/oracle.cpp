@@ -615,6 +615,13 @@ struct Counters {
   // walks' contributions (dropped either way).  The kernel does not trace
   // them; the tests compare its counters with these.
   uint64_t culled = 0, culled_nf = 0;
+  // Sources of the non-finite values (deviation 3: dropped, the path ends):
+  // scatter_surface estimators that are not finite, split into the rough
+  // BTDF with n_i == n_t (leaving a rough dielectric: refract returns the
+  // incident ray and the estimator is 0/0, main.py:1166-1185) and any other;
+  // of the former, those on a path's last segment (never read again); and
+  // non-finite NEE and emitter contributions.
+  uint64_t nf_btdf_eta1 = 0, nf_btdf_eta1_last = 0, nf_other_est = 0, nf_direct = 0, nf_indirect = 0;
 };
 static inline bool shadow_dense(float sigma_t) { return sigma_t * EPSILON >= 14.5f; }
 
@@ -1125,6 +1132,15 @@ static void cast_ray_path(const Scene& s, int W, int H, int i, int j, int band, 
         direct = ray.throughput * ns.emission * ns.contribution_factor * ns.weight;
         ray.path_length += 1;
         SurfaceEvent se = scatter_surface(s, ray, ev, rng);
+        if (!std::isfinite(se.estimator)) {
+          const Mat sm = get_mat(s, ev.hit_mat_id, ray.active), vm = get_mat(s, ray.vol_mat_id, ray.active);
+          if (!se.did_reflect && sm.tv && sm.r >= 0.0018f && vm.n == sm.n) {
+            cnt.nf_btdf_eta1++;
+            if (k == depth - 1) cnt.nf_btdf_eta1_last++;
+          } else {
+            cnt.nf_other_est++;
+          }
+        }
         ray.surface_pdf = se.pdf;
         ray.direction = se.direction;
         ray.throughput *= se.estimator;
@@ -1148,11 +1164,12 @@ static void cast_ray_path(const Scene& s, int W, int H, int i, int j, int band, 
         ray.path_length += 1;
       }
     }
-    if (std::isfinite(indirect)) acc[active_field] += indirect; else cnt.nonfinite++;
+    if (std::isfinite(indirect)) acc[active_field] += indirect; else { cnt.nonfinite++; cnt.nf_indirect++; }
     if (std::isfinite(direct)) {
       acc[active_field] += direct;
     } else {
       cnt.nonfinite++;
+      cnt.nf_direct++;
       if (nee_tp0) cnt.culled_nf++;
     }
   }
@@ -1297,13 +1314,15 @@ void or_trace(void* p, int n, const float* orig, const float* dir, float* out_t,
 // (average_and_reset_block main.py:1592-1607 applied once per block).
 // RNG key per path: c0 = i*H + j, c1 = block, c2 = band | (stream << 8).
 // stats = {segments, traversals, nonfinite, culled shadow traversals, culled
-// non-finite NEE terms} (the last two: see Counters).
+// non-finite NEE terms, then the sources of non-finite values: rough-BTDF
+// n_i == n_t estimators, those on a last segment, other estimators,
+// non-finite NEE terms, non-finite emitter terms} (see Counters).
 int or_render(void* p, int W, int H, uint64_t seed, uint32_t stream, int first_block, int n_blocks, int depth,
               int i0, int i1, float* accum, double* stats) {
   const Scene& s = *(Scene*)p;
   const int B = s.B;
-  uint64_t seg = 0, trav = 0, nf = 0, cul = 0, cnf = 0;
-#pragma omp parallel for schedule(dynamic, 1) reduction(+ : seg, trav, nf, cul, cnf)
+  uint64_t seg = 0, trav = 0, nf = 0, cul = 0, cnf = 0, e1 = 0, e1l = 0, eo = 0, nd = 0, ni = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : seg, trav, nf, cul, cnf, e1, e1l, eo, nd, ni)
   for (int i = i0; i < i1; ++i) {
     std::vector<float> blk(B);
     Counters cnt;
@@ -1324,10 +1343,14 @@ int or_render(void* p, int W, int H, uint64_t seed, uint32_t stream, int first_b
     }
     seg += cnt.segments; trav += cnt.traversals; nf += cnt.nonfinite;
     cul += cnt.culled; cnf += cnt.culled_nf;
+    e1 += cnt.nf_btdf_eta1; e1l += cnt.nf_btdf_eta1_last; eo += cnt.nf_other_est;
+    nd += cnt.nf_direct; ni += cnt.nf_indirect;
   }
   if (stats) {
     stats[0] = (double)seg; stats[1] = (double)trav; stats[2] = (double)nf;
     stats[3] = (double)cul; stats[4] = (double)cnf;
+    stats[5] = (double)e1; stats[6] = (double)e1l; stats[7] = (double)eo;
+    stats[8] = (double)nd; stats[9] = (double)ni;
   }
   return 0;
 }

@@ -147,11 +147,13 @@ class OracleScene:
         if accum is None:
             accum = np.zeros((width, height, B), np.float32)
         i0, i1 = (0, width) if cols is None else cols
-        stats = np.zeros(5, np.float64)
+        stats = np.zeros(10, np.float64)
         load().or_render(self.h, width, height, int(seed), int(stream), int(first_block), int(n_blocks), int(depth),
                          int(i0), int(i1), _p(accum), _p(stats))
         return accum, {"segments": int(stats[0]), "traversals": int(stats[1]), "nonfinite": int(stats[2]),
-                       "culled_traversals": int(stats[3]), "culled_nonfinite": int(stats[4])}
+                       "culled_traversals": int(stats[3]), "culled_nonfinite": int(stats[4]),
+                       "nf_btdf_eta1": int(stats[5]), "nf_btdf_eta1_last": int(stats[6]),
+                       "nf_other_estimator": int(stats[7]), "nf_direct": int(stats[8]), "nf_indirect": int(stats[9])}
 
     def shadow(self, o, d, max_dist, tri, vol, band):
         return float(load().or_shadow(self.h, _p(_f32(o)), _p(_f32(d)), float(max_dist), int(tri), int(vol),

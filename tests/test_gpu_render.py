@@ -86,6 +86,31 @@ def test_counters_match_cpu(box_small, gpu):
     assert abs(st["nonfinite"] - nf) <= 0.02 * max(nf, 50)
 
 
+@pytest.mark.parametrize("name,mat_id,roughness", [("volume_fluor", 6, None), ("box_icosphere", 5, 0.0017)])
+def test_nonfinite_counts_match_cpu(name, mat_id, roughness, gpu):
+    # The non-finite terms come from leaving a rough dielectric (a 0/0 BTDF
+    # estimator, main.py:1176-1185); the device drops and counts them as the
+    # oracle does.  With the sphere made smooth there are none on either side.
+    cfg = getattr(configs, name)()
+    scene = dict(cfg.scene)
+    if roughness is not None:
+        scene["materials"] = [dict(m) for m in scene["materials"]]
+        scene["materials"][mat_id]["roughness"] = [roughness] * cfg.bands
+    flat = flatten_scene(scene, cfg.bands, cfg.wl_min, cfg.wl_max)
+    W = H = 16
+    _, st0 = OracleScene(flat).render(W, H, cfg.seed, 0, 16, cfg.depth)
+    r = _renderer(flat, cfg, W, H, gpu)
+    r.render(n_blocks=16, max_depth=cfg.depth, seed=cfg.seed)
+    st = r.stats()
+    assert abs(st["segments"] - st0["segments"]) <= 0.005 * st0["segments"]
+    nf = st0["nonfinite"] - st0["culled_nonfinite"]
+    if roughness is None:
+        assert nf > 1000
+        assert abs(st["nonfinite"] - nf) <= 0.02 * nf
+    else:
+        assert nf == 0 and st["nonfinite"] == 0
+
+
 def test_independent_streams_statistical(box_small, gpu):
     cfg, flat, ora = box_small
     W = H = 8

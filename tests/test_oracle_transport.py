@@ -210,3 +210,63 @@ def test_running_mean_over_blocks():
     # blocks rendered alone reproduce the same paths (RNG keyed by block)
     assert np.allclose(acc, one, rtol=1e-5, atol=1e-7)
     assert np.allclose(np.mean(per, axis=0), one, rtol=1e-5, atol=1e-7)
+
+
+# ---------------------------------------------------------- non-finite terms
+def _nf_stats(cfg, w, n_blocks, roughness=None, mat_id=None, cols=None):
+    scene = dict(cfg.scene)
+    if roughness is not None:
+        scene["materials"] = [dict(m) for m in scene["materials"]]
+        scene["materials"][mat_id]["roughness"] = [roughness] * cfg.bands
+    flat = flatten_scene(scene, cfg.bands, cfg.wl_min, cfg.wl_max)
+    _, st = O.OracleScene(flat).render(w, w, cfg.seed, 0, n_blocks, cfg.depth, cols=cols)
+    return st
+
+
+# (config, render size, blocks, rough dielectric volume, non-finite events per
+# path segment measured on the oracle at this size, +-10%)
+NF_CASES = [("box_icosphere", 64, 4, 5, 0.0652), ("volume_fluor", 48, 2, 6, 0.0245)]
+
+
+@pytest.mark.parametrize("name,w,nb,mat_id,rate", NF_CASES)
+def test_nonfinite_source_is_rough_btdf_leaving_the_medium(name, w, nb, mat_id, rate):
+    # Leaving a rough (roughness >= 0.0018) true-volume dielectric, the
+    # reference takes n_i from the ray's volume and n_t from the hit surface,
+    # the same material: refract() returns the incident direction, L.H =
+    # -V.H, the denominator n_i V.H + n_t L.H is 0 and the BTDF estimator of
+    # main.py:1176-1185 is 0/0.  The path's throughput becomes NaN; it is
+    # counted (and the path ends) at the next segment, so an event on the
+    # last segment is never counted.  Every other source stays at zero.
+    cfg = getattr(configs, name)()
+    st = _nf_stats(cfg, w, nb)
+    assert st["nf_other_estimator"] == 0 and st["nf_direct"] == 0 and st["nf_indirect"] == 0
+    assert st["nf_btdf_eta1"] > 0
+    assert st["nonfinite"] == st["nf_btdf_eta1"] - st["nf_btdf_eta1_last"]
+    got = st["nonfinite"] / st["segments"]
+    assert abs(got - rate) <= 0.1 * rate, f"{got:.4f} non-finite events per segment"
+
+
+@pytest.mark.parametrize("name,w,nb,mat_id,rate", NF_CASES)
+def test_nonfinite_vanishes_for_a_smooth_dielectric(name, w, nb, mat_id, rate):
+    # Below roughness 0.0018 the transmission estimator is 1 - F
+    # (main.py:1167-1169): finite, so the same scene has no non-finite term;
+    # at 0.0018 the rough branch (and its 0/0) is taken again.  So near the
+    # threshold a second source appears: with alpha = roughness^2 the GGX
+    # D = alpha^2 / (pi denom^2) of main.py:900-907 and the terms built on
+    # it leave float range on some half vectors and the estimator is
+    # inf/inf; by roughness 0.02 it no longer fires.
+    cfg = getattr(configs, name)()
+    smooth = _nf_stats(cfg, w, nb, roughness=0.0017, mat_id=mat_id)
+    assert smooth["nonfinite"] == 0 and smooth["nf_btdf_eta1"] == 0
+    edge = _nf_stats(cfg, w, nb, roughness=0.0018, mat_id=mat_id)
+    assert edge["nf_btdf_eta1"] > 0 and edge["nf_other_estimator"] > 0
+    assert edge["nonfinite"] <= edge["nf_btdf_eta1"] + edge["nf_other_estimator"]
+    mid = _nf_stats(cfg, w, nb, roughness=0.02, mat_id=mat_id)
+    assert mid["nf_other_estimator"] == 0 and mid["nonfinite"] == mid["nf_btdf_eta1"] - mid["nf_btdf_eta1_last"]
+
+
+def test_heightfield_has_no_nonfinite_term():
+    # The heightfield is an open sheet: no path leaves a closed rough volume.
+    st = _nf_stats(configs.heightfield_1m(), 1024, 1, cols=(300, 316))
+    assert st["segments"] > 100000
+    assert st["nonfinite"] == 0 and st["nf_btdf_eta1"] == 0
