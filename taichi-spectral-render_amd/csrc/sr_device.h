@@ -242,9 +242,10 @@ __device__ __forceinline__ void visit_leaf(const float4* __restrict__ tris, int 
   if (cnt > 3) intersect_tri(o, d, a3, b3, c3, h.t, h.prim, h.u, h.v);
 }
 
-// Closest hit over the 4-wide BVH.  `stk` is this lane's LDS stack column
-// (entry i at stk[i * stride]).  Children are visited nearest-first; boxes
-// beyond the current best t are culled (ties kept: <=).
+// Closest hit over the 4-wide BVH, as a walk that can be suspended.  `stk`
+// is this lane's LDS stack column (entry i at stk[i * stride]).  Children
+// are visited nearest-first; boxes beyond the current best t are culled
+// (ties kept: <=).
 //
 // Loop structure (while-while with speculative traversal): a lane walks inner
 // nodes until it meets a leaf, parks that leaf and keeps walking nodes until
@@ -265,14 +266,42 @@ __device__ __forceinline__ void visit_leaf(const float4* __restrict__ tris, int 
 // exists before ESCAPE still decides between a volume event and an escape
 // (main.py:1210-1218).  The first far hit found shrinks the search to
 // [0, t_near], so a dense medium's traversal stops almost at once.
-__device__ __forceinline__ HitD trace_closest(const Node4* __restrict__ nodes, const float4* __restrict__ tris, f3 o,
-                                              f3 d, int* stk, int stride, uint32_t* overflow, float t_init = 1e9f,
-                                              float t_near = INFINITY) {
+//
+// A walk is the state below, started by trace_begin and advanced by
+// trace_resume, which the path loop calls once per iteration for every
+// lane whose ray is unfinished: a wave walks only while enough of its lanes
+// still walk, and the lanes whose rays ended meanwhile go on to shade and
+// start new rays that join the walk (the lanes' ballots decide when to
+// stop, so the wave's walks are compacted in time rather than in memory).
+// The state is one lane's: next node, parked leaf, stack depth, best hit
+// so far and the exact-hit limit.
+struct TraceState {
+  int cur, leaf, sp;
   HitD h;
-  h.t = t_init;  // HitRecord(t=1e9), main.py:833
-  h.prim = INT32_MIN;
-  h.u = 0.f;
-  h.v = 0.f;
+  float t_near;
+  bool far;  // a hit in (t_near, t_init) was seen
+};
+
+__device__ __forceinline__ void trace_begin(TraceState& s, float t_init, float t_near) {
+  s.cur = 0;  // root node; SR_EMPTY = nothing left
+  s.leaf = SR_EMPTY;
+  s.sp = 0;
+  s.h.t = t_init;  // HitRecord(t=1e9), main.py:833
+  s.h.prim = INT32_MIN;
+  s.h.u = 0.f;
+  s.h.v = 0.f;
+  s.t_near = t_near;
+  s.far = false;
+}
+
+// Walks the rays of the lanes with `live` set, in rounds of the
+// while-while loop below, until fewer than min(keep, walking lanes at entry)
+// lanes of the wave are still walking (keep <= 1: every walk ends).  A lane
+// whose walk ended has its final hit in s.h and returns true; the others
+// keep their state (and their stack column) for the next call.
+__device__ __forceinline__ bool trace_resume(const Node4* __restrict__ nodes, const float4* __restrict__ tris, f3 o,
+                                             f3 d, int* stk, int stride, uint32_t* overflow, TraceState& s, bool live,
+                                             int keep) {
   const float tiny = 1e-12f;
   f3 dd = mk(fabsf(d.x) < tiny ? copysignf(tiny, d.x) : d.x, fabsf(d.y) < tiny ? copysignf(tiny, d.y) : d.y,
              fabsf(d.z) < tiny ? copysignf(tiny, d.z) : d.z);
@@ -280,12 +309,18 @@ __device__ __forceinline__ HitD trace_closest(const Node4* __restrict__ nodes, c
   f3 oi = mk(o.x * inv.x, o.y * inv.y, o.z * inv.z);
   // the near plane of an axis is the box's hi side when the direction is negative
   const bool nx = inv.x < 0.f, ny = inv.y < 0.f, nz = inv.z < 0.f;
-  int sp = 0;
-  int cur = 0;       // root node; SR_EMPTY = nothing left
-  int leaf = SR_EMPTY;  // parked leaf
+  int cur = live ? s.cur : SR_EMPTY;
+  int leaf = live ? s.leaf : SR_EMPTY;
+  int sp = s.sp;
+  HitD h = s.h;
+  bool far = s.far;
+  const float t_near = s.t_near;
   const float t_near_up = nextafterf(t_near, INFINITY);
-  bool far = false;  // a hit in (t_near, t_init) was seen
-  while (cur != SR_EMPTY || leaf != SR_EMPTY) {
+  bool walking = cur != SR_EMPTY || leaf != SR_EMPTY;
+  const int n0 = __popcll(__ballot(walking));
+  const int stop = min(keep, n0);  // leave once fewer than this still walk
+  int n = n0;
+  while (n > 0) {
     // node loop
     while (cur != SR_EMPTY && cur >= 0) {
       int next;
@@ -313,9 +348,29 @@ __device__ __forceinline__ HitD trace_closest(const Node4* __restrict__ nodes, c
         cur = sp > 0 ? stk[(--sp) * stride] : SR_EMPTY;
       }
     }
+    walking = cur != SR_EMPTY || leaf != SR_EMPTY;
+    n = __popcll(__ballot(walking));
+    if (n < stop) break;
   }
-  if (h.prim == INT32_MIN) h.prim = far ? -2 : -1;
-  return h;
+  if (live) {
+    s.cur = cur;
+    s.leaf = leaf;
+    s.sp = sp;
+    s.far = far;
+    if (!walking && h.prim == INT32_MIN) h.prim = far ? -2 : -1;
+    s.h = h;
+  }
+  return live && !walking;
+}
+
+// The closest hit, walked to the end in one call.
+__device__ __forceinline__ HitD trace_closest(const Node4* __restrict__ nodes, const float4* __restrict__ tris, f3 o,
+                                              f3 d, int* stk, int stride, uint32_t* overflow, float t_init = 1e9f,
+                                              float t_near = INFINITY) {
+  TraceState s;
+  trace_begin(s, t_init, t_near);
+  trace_resume(nodes, tris, o, d, stk, stride, overflow, s, true, 0);
+  return s.h;
 }
 
 // HitRecord completion (main.py:860-863) from the face's shading record.

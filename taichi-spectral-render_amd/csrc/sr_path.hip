@@ -1035,7 +1035,48 @@ __device__ __forceinline__ void flush_counters(const PathArgs& P, int j, Lane& L
   L.n_seg = L.n_trav = L.n_nf = 0;
 }
 
+// The walk limits of the lane's ray: extension rays report the exact
+// closest hit up to the free flight (t_near) and otherwise only whether a
+// surface lies before ESCAPE; shadow steps may start their search at the
+// light when the medium cannot attenuate.
+// A suspended walk goes on in the next iteration while at least this many
+// lanes of the wave still walk.
+#ifndef SR_TRACE_KEEP
+#define SR_TRACE_KEEP 24
+#endif
+
 template <bool ADJ, bool SSS>
+__device__ __forceinline__ void ray_limits(const Lane& L, const Ctx<ADJ>& c, const PathArgs& P, float& t_init,
+                                           float& t_near) {
+  t_init = 1e9f;
+  t_near = INFINITY;
+  if (L.phase == PH_EXT) {
+    if (SSS && L.walk == 1) {
+      t_init = nextafterf(L.smax, INFINITY);  // hits at t <= the flight
+    } else {
+      // a path segment: hits at or beyond ESCAPE mean "escaped"
+      // (main.py:1210), and past the free flight only their existence counts
+      MatRec vm;
+      float sa;
+      U4 re4;
+      t_init = SR_ESCAPE;
+      t_near = ext_flight<ADJ>(L, c, sa, vm, re4) + SR_EPSILON;
+    }
+  } else {
+    // A shadow step through a medium with sigma_a + sigma_s = 0 needs only
+    // the hits before the light: any hit at t >= smax - strav ends the walk
+    // as "reached" with an attenuation of exactly 1 whatever its t
+    // (main.py:1463-1468), so the search starts at that distance.  (Not for
+    // the adjoint's sigma_a material: its span derivative uses t.)  (k = 0
+    // and sigma_s = 0 make that sum exactly 0.)
+    const SceneDev& s = c.a.s;
+    const float4 mv = *reinterpret_cast<const float4*>(s.mats + L.svol * s.B + L.active);
+    if (mv.y == 0.0f && mv.w == 0.0f && !(ADJ && L.svol == P.adj.sig_mat)) t_init = L.smax - L.strav;
+  }
+}
+
+// REFILL picks the wave's traversal discipline (see the loop body).
+template <bool ADJ, bool SSS, bool REFILL>
 __device__ __forceinline__ void path_loop(const PathArgs& P, int* stk, float* lds_grad) {
   const int stride = blockDim.x;
   const int lane = (int)__lane_id();
@@ -1058,6 +1099,9 @@ __device__ __forceinline__ void path_loop(const PathArgs& P, int* stk, float* ld
   L.fdirect = 0.0f;
   bool has_item = false;
   int band_end = 0;  // the lane's item covers bands [.., band_end)
+  int tr = 0;  // the lane's ray: 0 not started, 1 walking, 2 hit known
+  TraceState ts;
+  trace_begin(ts, 0.0f, 0.0f);
   // The job table is written only by the host: read it through the constant
   // address space so every field access is a (cached) scalar load.
   typedef __attribute__((address_space(4))) const Job ConstJob;
@@ -1133,76 +1177,110 @@ __device__ __forceinline__ void path_loop(const PathArgs& P, int* stk, float* ld
       lbase = lend = 0;
       continue;
     }
-    // One handler group per iteration, chosen for the whole wave: extension
-    // rays (trace + scatter_volume + NEE set-up) or shadow-walk steps (trace
-    // + walk step + end of segment).  Lanes of the other group wait one
-    // iteration.  Every path alternates EXT -> SHD -> FIN -> EXT, so the
-    // lanes fall into step and each handler runs once for most of the wave,
-    // instead of once per divergent subset in every iteration.  A streak
-    // limit keeps a few shadow lanes from waiting behind a wave whose
-    // extension rays keep restarting (escaping paths).
-    const int n_ext = __popcll(__ballot(L.phase == PH_EXT));
-    const int n_shd = __popcll(__ballot(L.phase == PH_SHD || L.phase == PH_FIN));
-    const bool do_ext = n_ext > n_shd && !(ext_streak >= 4 && n_shd > 0);
-    ext_streak = do_ext ? ext_streak + 1 : 0;
-    if (L.phase == (do_ext ? PH_EXT : PH_SHD)) {
-      f3 ro = do_ext ? L.pos : L.pos + L.sdir * L.soff;
-      f3 rd = do_ext ? L.dir : L.sdir;
-      float t_init = 1e9f, t_near = INFINITY;
-      if (SSS && do_ext && L.walk == 1) {
-        t_init = nextafterf(L.smax, INFINITY);  // hits at t <= the flight
-      } else if (do_ext) {
-        // a path segment: hits at or beyond ESCAPE mean "escaped"
-        // (main.py:1210), and past the free flight only their existence counts
-        MatRec vm;
-        float sa;
-        U4 re4;
-        t_init = SR_ESCAPE;
-        t_near = ext_flight<ADJ>(L, c, sa, vm, re4) + SR_EPSILON;
+    bool do_ext;
+    if (REFILL) {
+      // Rays.  A lane in PH_EXT or PH_SHD owns one ray: tr is 0 before its
+      // walk starts, 1 while it walks the BVH, 2 once its hit is known.
+      // Every walking lane of the wave walks together, extension and shadow
+      // rays alike, until fewer than SR_TRACE_KEEP of them still walk; a
+      // walk that outlasts the iteration keeps its state (and its LDS stack
+      // column) and goes on in the next, beside the rays that the finished
+      // lanes start after their handlers.
+      if ((L.phase == PH_EXT || L.phase == PH_SHD) && tr == 0) {
+        float t_init, t_near;
+        ray_limits<ADJ, SSS>(L, c, P, t_init, t_near);
+        trace_begin(ts, t_init, t_near);
+        tr = 1;
       }
-      if (!do_ext) {
-        // A shadow step through a medium with sigma_a + sigma_s = 0 needs
-        // only the hits before the light: any hit at t >= smax - strav ends
-        // the walk as "reached" with an attenuation of exactly 1 whatever
-        // its t (main.py:1463-1468), so the search starts at that distance.
-        // (Not for the adjoint's sigma_a material: its span derivative uses t.)
-        // (k = 0 and sigma_s = 0 make that sum exactly 0.)
-        const float4 mv = *reinterpret_cast<const float4*>(J.s.mats + L.svol * J.s.B + L.active);
-        if (mv.y == 0.0f && mv.w == 0.0f && !(ADJ && L.svol == P.adj.sig_mat)) t_init = L.smax - L.strav;
+      if (__ballot(tr == 1)) {
+        const bool ext = L.phase == PH_EXT;
+        const f3 ro = ext ? L.pos : L.pos + L.sdir * L.soff;
+        const f3 rd = ext ? L.dir : L.sdir;
+        if (trace_resume(J.s.nodes, J.s.tris, ro, rd, stk, stride, P.overflow, ts, tr == 1, SR_TRACE_KEEP)) tr = 2;
       }
-      const Node4* nodes = J.s.nodes;
-      const float4* tris = J.s.tris;
-      HitD h = trace_closest(nodes, tris, ro, rd, stk, stride, P.overflow, t_init, t_near);
-      if (do_ext) on_extension<ADJ, SSS>(L, c, h);
-      else on_shadow<ADJ, SSS>(L, c, h);
+      // One handler group per iteration, chosen for the whole wave among
+      // the lanes whose ray is done: extension rays (scatter_volume + NEE
+      // set-up) or shadow-walk steps (walk step + end of segment).  Ready
+      // lanes of the other group wait.  A streak limit keeps a few shadow
+      // lanes from waiting behind a wave whose extension rays keep
+      // restarting (escaping paths).
+      const int n_ext = __popcll(__ballot(L.phase == PH_EXT && tr == 2));
+      const int n_shd = __popcll(__ballot((L.phase == PH_SHD && tr == 2) || L.phase == PH_FIN));
+      if (n_ext + n_shd == 0) continue;
+      do_ext = n_ext > n_shd && !(ext_streak >= 4 && n_shd > 0);
+      ext_streak = do_ext ? ext_streak + 1 : 0;
+      if (tr == 2 && L.phase == (do_ext ? PH_EXT : PH_SHD)) {
+        tr = 0;
+        if (do_ext) on_extension<ADJ, SSS>(L, c, ts.h);
+        else on_shadow<ADJ, SSS>(L, c, ts.h);
+      }
+    } else {
+      // One handler group per iteration, chosen for the whole wave:
+      // extension rays (trace + scatter_volume + NEE set-up) or shadow-walk
+      // steps (trace + walk step + end of segment), each ray walked to its
+      // end.  Lanes of the other group wait one iteration.  Every path
+      // alternates EXT -> SHD -> FIN -> EXT, so the lanes fall into step.
+      const int n_ext = __popcll(__ballot(L.phase == PH_EXT));
+      const int n_shd = __popcll(__ballot(L.phase == PH_SHD || L.phase == PH_FIN));
+      do_ext = n_ext > n_shd && !(ext_streak >= 4 && n_shd > 0);
+      ext_streak = do_ext ? ext_streak + 1 : 0;
+      if (L.phase == (do_ext ? PH_EXT : PH_SHD)) {
+        float t_init, t_near;
+        ray_limits<ADJ, SSS>(L, c, P, t_init, t_near);
+        const f3 ro = do_ext ? L.pos : L.pos + L.sdir * L.soff;
+        const f3 rd = do_ext ? L.dir : L.sdir;
+        const HitD h = trace_closest(J.s.nodes, J.s.tris, ro, rd, stk, stride, P.overflow, t_init, t_near);
+        if (do_ext) on_extension<ADJ, SSS>(L, c, h);
+        else on_shadow<ADJ, SSS>(L, c, h);
+      }
     }
     if (!do_ext && L.phase == PH_FIN) finish_segment<ADJ, SSS>(L, c, L.fdirect);
   }
 }
 
-// The traversal is bound by the latency of its dependent node fetches, so
-// the kernel asks for 4 waves per SIMD (128 VGPRs; the compiler spills a
-// few long-lived shading values, none inside the traversal loops).  The
-// 40-entry LDS stack of 256-lane blocks also fits 4 blocks per CU.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void sr_path_kernel(PathArgs a) {
+// Traversal discipline and occupancy of the three path kernels, chosen by
+// measurement on the 1M-triangle heightfield and the volume scenes.  The
+// walk state a suspended ray keeps (TraceState, ~9 registers) does not fit
+// beside the shading state at 4 waves per SIMD (128 VGPRs: the spills cost
+// more than the refill gains), so the kernels run at 3 waves per SIMD
+// (168 VGPRs; 12 waves, three 256-lane blocks and their 40-entry LDS stacks
+// per CU).  The subsurface kernel keeps the walk-to-the-end loop: its
+// random walk gains nothing from the refill.
+#ifndef SR_REFILL_PATH
+#define SR_REFILL_PATH true
+#endif
+#ifndef SR_REFILL_SSS
+#define SR_REFILL_SSS false
+#endif
+#ifndef SR_REFILL_ADJ
+#define SR_REFILL_ADJ true
+#endif
+#ifndef SR_WAVES_SSS
+#define SR_WAVES_SSS 3
+#endif
+#ifndef SR_WAVES_ADJ
+#define SR_WAVES_ADJ 3
+#endif
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void sr_path_kernel(PathArgs a) {
   extern __shared__ int lds_dyn[];
   if (threadIdx.x == 0) atomicMin(a.clocks, wall_clock64());
-  path_loop<false, false>(a, lds_dyn + threadIdx.x, nullptr);
+  path_loop<false, false, SR_REFILL_PATH>(a, lds_dyn + threadIdx.x, nullptr);
 }
 
 // The same path loop with random-walk subsurface scattering (SR_RENDER_SUBSURFACE).
-__global__ __launch_bounds__(256) void sr_path_sss_kernel(PathArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SR_WAVES_SSS, SR_WAVES_SSS))) void sr_path_sss_kernel(PathArgs a) {
   extern __shared__ int lds_dyn[];
   if (threadIdx.x == 0) atomicMin(a.clocks, wall_clock64());
-  path_loop<false, true>(a, lds_dyn + threadIdx.x, nullptr);
+  path_loop<false, true, SR_REFILL_SSS>(a, lds_dyn + threadIdx.x, nullptr);
 }
 
-__global__ __launch_bounds__(256) void sr_adjoint_kernel(PathArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SR_WAVES_ADJ, SR_WAVES_ADJ))) void sr_adjoint_kernel(PathArgs a) {
   extern __shared__ int lds_dyn[];
   float* lds_grad = reinterpret_cast<float*>(lds_dyn + SR_STACK_SIZE * blockDim.x);
   for (int i = threadIdx.x; i < a.adj.n_params; i += blockDim.x) lds_grad[i] = 0.0f;
   __syncthreads();
-  path_loop<true, false>(a, lds_dyn + threadIdx.x, lds_grad);
+  path_loop<true, false, SR_REFILL_ADJ>(a, lds_dyn + threadIdx.x, lds_grad);
   __syncthreads();
   for (int i = threadIdx.x; i < a.adj.n_params; i += blockDim.x) {
     float v = lds_grad[i];

@@ -292,12 +292,13 @@ def test_empty_scene_renders_black(gpu):
 
 def test_volume_fluorescence_matched_seed_parity(gpu):
     """The HG medium + fluorescent slab scene (volume fluorescence shifts
-    bands) at reduced resolution: matched-seed parity as above."""
+    bands) at reduced resolution: matched-seed parity as above, the same
+    gate at 4096 blocks."""
     cfg = configs.volume_fluor()
     flat = flatten_scene(cfg.scene, cfg.bands, cfg.wl_min, cfg.wl_max)
     ora = OracleScene(flat)
-    W = H = 8
-    n_chunks, per = 8, 256
+    W = H = 12
+    n_chunks, per = 8, 512  # 4096 spp
     cpu = _chunks_oracle(ora, W, H, cfg.seed, cfg.depth, n_chunks, per)
     r = _renderer(flat, cfg, W, H, gpu)
     g = _chunks_gpu(r, cfg.seed, cfg.depth, n_chunks, per)
