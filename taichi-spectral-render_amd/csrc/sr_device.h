@@ -282,6 +282,12 @@ struct TraceState {
   bool far;  // a hit in (t_near, t_init) was seen
 };
 
+// The path kernels suspend a wave's walks once fewer than this many of its
+// lanes still walk.
+#ifndef SR_TRACE_KEEP
+#define SR_TRACE_KEEP 24
+#endif
+
 __device__ __forceinline__ void trace_begin(TraceState& s, float t_init, float t_near) {
   s.cur = 0;  // root node; SR_EMPTY = nothing left
   s.leaf = SR_EMPTY;

@@ -16,7 +16,16 @@ __global__ __launch_bounds__(256) void sr_trace_kernel(SceneDev s, int n, const 
   if (r >= n) return;
   f3 o = mk(orig[r * 3], orig[r * 3 + 1], orig[r * 3 + 2]);
   f3 d = mk(dir[r * 3], dir[r * 3 + 1], dir[r * 3 + 2]);
-  HitD h = trace_closest(s.nodes, s.tris, o, d, lds_stack + threadIdx.x, blockDim.x, overflow);
+  // walked in suspended pieces, as the path kernels walk theirs: a piece
+  // ends once fewer than SR_TRACE_KEEP lanes of the wave still walk, and the
+  // next resumes the unfinished walks (the hit equals an uninterrupted walk's)
+  TraceState ts;
+  trace_begin(ts, 1e9f, INFINITY);
+  bool done = false;
+  while (__ballot(!done))
+    if (trace_resume(s.nodes, s.tris, o, d, lds_stack + threadIdx.x, blockDim.x, overflow, ts, !done, SR_TRACE_KEEP))
+      done = true;
+  const HitD h = ts.h;
   out_t[r] = h.t;
   out_prim[r] = h.prim;
   out_uv[r * 2] = h.prim >= 0 ? h.u : 0.0f;

@@ -1039,12 +1039,6 @@ __device__ __forceinline__ void flush_counters(const PathArgs& P, int j, Lane& L
 // closest hit up to the free flight (t_near) and otherwise only whether a
 // surface lies before ESCAPE; shadow steps may start their search at the
 // light when the medium cannot attenuate.
-// A suspended walk goes on in the next iteration while at least this many
-// lanes of the wave still walk.
-#ifndef SR_TRACE_KEEP
-#define SR_TRACE_KEEP 24
-#endif
-
 template <bool ADJ, bool SSS>
 __device__ __forceinline__ void ray_limits(const Lane& L, const Ctx<ADJ>& c, const PathArgs& P, float& t_init,
                                            float& t_near) {
