@@ -209,8 +209,6 @@ def bench_batch(spp_scale=8, graph=True, log=print, build="host"):
         evs.append(e)
     torch.cuda.synchronize()
     lat = [start.elapsed_time(e) for e in evs]
-    for _, _, r in rs:
-        pass
     seq_ms = lat[-1]
     sequential = {"ms": round(seq_ms, 2), "Mseg_per_s": round(segs / seq_ms / 1e3, 1),
                   "p50_latency_ms": round(float(np.percentile(lat, 50)), 2),
