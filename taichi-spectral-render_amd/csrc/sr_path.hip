@@ -1192,22 +1192,23 @@ __device__ __forceinline__ void path_loop(const PathArgs& P, int* stk, float* ld
         const f3 rd = ext ? L.dir : L.sdir;
         if (trace_resume(J.s.nodes, J.s.tris, ro, rd, stk, stride, P.overflow, ts, tr == 1, SR_TRACE_KEEP)) tr = 2;
       }
-      // One handler group per iteration, chosen for the whole wave among
-      // the lanes whose ray is done: extension rays (scatter_volume + NEE
-      // set-up) or shadow-walk steps (walk step + end of segment).  Ready
-      // lanes of the other group wait.  A streak limit keeps a few shadow
-      // lanes from waiting behind a wave whose extension rays keep
-      // restarting (escaping paths).
-      const int n_ext = __popcll(__ballot(L.phase == PH_EXT && tr == 2));
-      const int n_shd = __popcll(__ballot((L.phase == PH_SHD && tr == 2) || L.phase == PH_FIN));
-      if (n_ext + n_shd == 0) continue;
-      do_ext = n_ext > n_shd && !(ext_streak >= 4 && n_shd > 0);
-      ext_streak = do_ext ? ext_streak + 1 : 0;
-      if (tr == 2 && L.phase == (do_ext ? PH_EXT : PH_SHD)) {
+      // Every lane whose ray is done shades in this iteration: the
+      // extension handler (scatter_volume + NEE set-up) for the extension
+      // rays, then the shadow-walk handler (walk step) for the shadow rays,
+      // then the end of segment (incl. scatter_surface) for the lanes whose
+      // walk is over.  Waiting a ready group out for a larger one (the
+      // walk-to-the-end loop below) only delays lanes here: the walks do
+      // not wait for the handlers, so a lane held back is a lane idle.
+      if (__ballot(tr == 2 || L.phase == PH_FIN) == 0ull) continue;
+      if (tr == 2 && L.phase == PH_EXT) {
         tr = 0;
-        if (do_ext) on_extension<ADJ, SSS>(L, c, ts.h);
-        else on_shadow<ADJ, SSS>(L, c, ts.h);
+        on_extension<ADJ, SSS>(L, c, ts.h);
       }
+      if (tr == 2 && L.phase == PH_SHD) {
+        tr = 0;
+        on_shadow<ADJ, SSS>(L, c, ts.h);
+      }
+      do_ext = false;  // (every lane at PH_FIN ends its segment below)
     } else {
       // One handler group per iteration, chosen for the whole wave:
       // extension rays (trace + scatter_volume + NEE set-up) or shadow-walk
