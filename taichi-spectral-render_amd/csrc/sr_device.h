@@ -65,10 +65,14 @@ __device__ __forceinline__ f3 xyz(float4 v) { return mk(v.x, v.y, v.z); }
 // at one place in the code, for every lane that reaches it.
 __device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
                                        uint32_t out[4]) {
+  // Each round's two 32 x 32 -> 64-bit products are formed whole (one
+  // v_mad_u64_u32 each) rather than as separate high and low multiplies:
+  // the same bits for half the quarter-rate multiplies.
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
-    uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
-    uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
     uint32_t n0 = hi1 ^ c1 ^ k0;
     uint32_t n2 = hi0 ^ c3 ^ k1;
     c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
@@ -151,6 +155,9 @@ __device__ __forceinline__ void intersect_tri(f3 o, f3 d, float4 a, float4 b, fl
   if (ok) { best_t = t; best_prim = prim; bu = u; bv = v; }
 }
 
+// Entry i (0 <= i < SR_STACK_SIZE) of a lane's LDS stack column.
+__device__ __forceinline__ int& stk_at(int* stk, int stride, int i) { return stk[i * stride]; }
+
 // Tests the 4 children of node `cur` against the ray; returns the nearest
 // hit child in `next` (SR_EMPTY when none) and pushes the others far-to-near.
 // The near and far planes of each axis are fetched by address -- `ox`, `oy`,
@@ -195,10 +202,6 @@ __device__ __forceinline__ void visit_node(const Node4* __restrict__ nodes, int 
     key[c] = hit ? tn[c] : INFINITY;
     hitc += hit ? 1 : 0;
   }
-  if (hitc == 0) {
-    next = SR_EMPTY;
-    return;
-  }
 #define SR_CSWAP(i, j)                                                   \
   {                                                                      \
     const bool sw = key[j] < key[i];                                     \
@@ -209,18 +212,23 @@ __device__ __forceinline__ void visit_node(const Node4* __restrict__ nodes, int 
   }
   SR_CSWAP(0, 1) SR_CSWAP(2, 3) SR_CSWAP(0, 2) SR_CSWAP(1, 3) SR_CSWAP(1, 2)
 #undef SR_CSWAP
-  next = cref[0];
-  // push the other hits far-to-near: cref[c] goes to sp + np - c, so the
-  // nearest of them ends on top (the builder bounds the depth so that the
-  // stack cannot overflow; an overflow is still flagged, never written)
-  const int np = hitc - 1;
-  if (sp + np > SR_STACK_SIZE) atomicOr(overflow, 1u);
-#pragma unroll
-  for (int c = 1; c <= 3; ++c) {
-    const int slot = sp + np - c;
-    if (c <= np && slot < SR_STACK_SIZE) stk[slot * stride] = cref[c];
-  }
-  sp = min(sp + np, SR_STACK_SIZE);
+  next = hitc > 0 ? cref[0] : SR_EMPTY;
+  // Push the other hits far-to-near with three unconditional writes to
+  // slots sp, sp + 1, sp + 2: slot sp + k takes the (np - k)-th nearest of
+  // them while k < np, so the nearest ends on top; the slots at and above
+  // the new top take copies that are never read.  No branch, whatever the
+  // lane's hit count.  The builder bounds the depth so that a visit starts
+  // with sp <= SR_STACK_SIZE - 3 (three entries per level above it); a
+  // deeper stack is flagged (never written out of the lane's column).
+  const int np = max(hitc - 1, 0);
+  if (sp > SR_STACK_SIZE - 3) atomicOr(overflow, 1u);
+  const int s0 = min(sp, SR_STACK_SIZE - 3);
+  const int w0 = np == 3 ? cref[3] : (np == 2 ? cref[2] : cref[1]);
+  const int w1 = np == 3 ? cref[2] : cref[1];
+  stk_at(stk, stride, s0) = w0;
+  stk_at(stk, stride, s0 + 1) = w1;
+  stk_at(stk, stride, s0 + 2) = cref[1];
+  sp = s0 + np;
 }
 
 // All SR_LEAF_MAX (4) triangle records of a leaf are fetched before any is
@@ -331,12 +339,24 @@ __device__ __forceinline__ bool trace_resume(const Node4* __restrict__ nodes, co
     while (cur != SR_EMPTY && cur >= 0) {
       int next;
       visit_node(nodes, cur, inv, oi, nx, ny, nz, h.t, stk, stride, sp, next, overflow);
-      if (next == SR_EMPTY) next = sp > 0 ? stk[(--sp) * stride] : SR_EMPTY;
+      // The two entries below the top are read unconditionally (a slot
+      // below 0 reads slot 0 and is not used): with no child to go on with,
+      // the walk pops the top; when the node it goes on with is the first
+      // leaf, it parks that leaf and pops (again).
+      const int t1 = stk_at(stk, stride, max(sp - 1, 0)), t2 = stk_at(stk, stride, max(sp - 2, 0));
+      const bool popped = next == SR_EMPTY;
+      int s1 = sp;
+      if (popped) {
+        next = sp > 0 ? t1 : SR_EMPTY;
+        s1 = max(sp - 1, 0);
+      }
       cur = next;
       if (cur != SR_EMPTY && cur < 0 && leaf == SR_EMPTY) {  // park the first leaf, keep walking
         leaf = cur;
-        cur = sp > 0 ? stk[(--sp) * stride] : SR_EMPTY;
+        cur = s1 > 0 ? (popped ? t2 : t1) : SR_EMPTY;
+        s1 = max(s1 - 1, 0);
       }
+      sp = s1;
       const bool waiting = cur != SR_EMPTY && cur >= 0 && leaf == SR_EMPTY;
       if (__ballot(waiting) == 0ull) break;
     }
@@ -351,7 +371,9 @@ __device__ __forceinline__ bool trace_resume(const Node4* __restrict__ nodes, co
       leaf = SR_EMPTY;
       if (cur != SR_EMPTY && cur < 0) {
         leaf = cur;
-        cur = sp > 0 ? stk[(--sp) * stride] : SR_EMPTY;
+        const int top = stk_at(stk, stride, max(sp - 1, 0));
+        cur = sp > 0 ? top : SR_EMPTY;
+        sp = max(sp - 1, 0);
       }
     }
     walking = cur != SR_EMPTY || leaf != SR_EMPTY;
