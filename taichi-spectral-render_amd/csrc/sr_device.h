@@ -150,9 +150,13 @@ __device__ __forceinline__ void intersect_tri(f3 o, f3 d, float4 a, float4 b, fl
   f3 qvec = cross(tvec, e1);
   float v = dot(d, qvec) * inv_det;
   float t = dot(e2, qvec) * inv_det;
-  bool ok = fabsf(det) > 1e-8f && u >= 0.0f && u <= 1.0f && v >= 0.0f && u + v <= 1.0f && t > SR_EPSILON;
-  ok = ok && (t < best_t || (t == best_t && prim < best_prim));
-  if (ok) { best_t = t; best_prim = prim; bu = u; bv = v; }
+  // bitwise, not short-circuit, tests and a select: no branch
+  const bool ok = (fabsf(det) > 1e-8f) & (u >= 0.0f) & (u <= 1.0f) & (v >= 0.0f) & (u + v <= 1.0f) &
+                  (t > SR_EPSILON) & ((t < best_t) | ((t == best_t) & (prim < best_prim)));
+  best_t = ok ? t : best_t;
+  best_prim = ok ? prim : best_prim;
+  bu = ok ? u : bu;
+  bv = ok ? v : bv;
 }
 
 // Entry i (0 <= i < SR_STACK_SIZE) of a lane's LDS stack column.
@@ -169,7 +173,7 @@ __device__ __forceinline__ int& stk_at(int* stk, int stride, int i) { return stk
 // the order (ties in t go to the smaller face id).
 __device__ __forceinline__ void visit_node(const Node4* __restrict__ nodes, int cur, f3 inv, f3 oi, bool nx, bool ny,
                                            bool nz, float tmax_, int* stk, int stride, int& sp, int& next,
-                                           uint32_t* overflow) {
+                                           bool& ovf) {
   // 32-bit byte offsets from the array base (kept in scalar registers): one
   // VGPR per address instead of a 64-bit pointer
   const char* base = reinterpret_cast<const char*>(nodes);
@@ -221,7 +225,7 @@ __device__ __forceinline__ void visit_node(const Node4* __restrict__ nodes, int 
   // with sp <= SR_STACK_SIZE - 3 (three entries per level above it); a
   // deeper stack is flagged (never written out of the lane's column).
   const int np = max(hitc - 1, 0);
-  if (sp > SR_STACK_SIZE - 3) atomicOr(overflow, 1u);
+  ovf = ovf || sp > SR_STACK_SIZE - 3;
   const int s0 = min(sp, SR_STACK_SIZE - 3);
   const int w0 = np == 3 ? cref[3] : (np == 2 ? cref[2] : cref[1]);
   const int w1 = np == 3 ? cref[2] : cref[1];
@@ -231,23 +235,25 @@ __device__ __forceinline__ void visit_node(const Node4* __restrict__ nodes, int 
   sp = s0 + np;
 }
 
-// All SR_LEAF_MAX (4) triangle records of a leaf are fetched before any is
-// tested, so a leaf costs one memory round trip whatever its size (the
-// triangle array ends with SR_LEAF_MAX - 1 zero records, so the reads stay
-// in bounds).  The result does not depend on the test order (ties go to the
-// smaller face id).
+// All SR_LEAF_MAX (4) triangle records from the leaf's first are fetched
+// and tested, whatever the leaf's count: a leaf costs one memory round trip
+// and no branch.  Records past the leaf's own faces are faces of the scene
+// (the next leaves' in leaf order) or the SR_LEAF_MAX - 1 zero records at
+// the end of the array (det = 0: never a hit), so they cannot change the
+// walk's answer: a hit on such a face lies inside that face's own leaf box,
+// which the walk visits unless a closer hit culls it.  The result does not
+// depend on the test order (ties go to the smaller face id).
 static_assert(SR_LEAF_MAX == 4, "visit_leaf reads four triangle records");
 __device__ __forceinline__ void visit_leaf(const float4* __restrict__ tris, int ref, f3 o, f3 d, HitD& h) {
   const uint32_t r = (uint32_t)ref;
   const int first = (int)((r & ~SR_LEAF_BIT) >> 3);
-  const int cnt = (int)(r & 7u);
   const float4* tp = tris + (size_t)first * 3;
   const float4 a0 = tp[0], b0 = tp[1], c0 = tp[2], a1 = tp[3], b1 = tp[4], c1 = tp[5];
   const float4 a2 = tp[6], b2 = tp[7], c2 = tp[8], a3 = tp[9], b3 = tp[10], c3 = tp[11];
   intersect_tri(o, d, a0, b0, c0, h.t, h.prim, h.u, h.v);
-  if (cnt > 1) intersect_tri(o, d, a1, b1, c1, h.t, h.prim, h.u, h.v);
-  if (cnt > 2) intersect_tri(o, d, a2, b2, c2, h.t, h.prim, h.u, h.v);
-  if (cnt > 3) intersect_tri(o, d, a3, b3, c3, h.t, h.prim, h.u, h.v);
+  intersect_tri(o, d, a1, b1, c1, h.t, h.prim, h.u, h.v);
+  intersect_tri(o, d, a2, b2, c2, h.t, h.prim, h.u, h.v);
+  intersect_tri(o, d, a3, b3, c3, h.t, h.prim, h.u, h.v);
 }
 
 // Closest hit over the 4-wide BVH, as a walk that can be suspended.  `stk`
@@ -331,6 +337,7 @@ __device__ __forceinline__ bool trace_resume(const Node4* __restrict__ nodes, co
   const float t_near = s.t_near;
   const float t_near_up = nextafterf(t_near, INFINITY);
   bool walking = cur != SR_EMPTY || leaf != SR_EMPTY;
+  bool ovf = false;  // the lane's stack outgrew its column (flagged once, after the rounds)
   const int n0 = __popcll(__ballot(walking));
   const int stop = min(keep, n0);  // leave once fewer than this still walk
   int n = n0;
@@ -338,7 +345,7 @@ __device__ __forceinline__ bool trace_resume(const Node4* __restrict__ nodes, co
     // node loop
     while (cur != SR_EMPTY && cur >= 0) {
       int next;
-      visit_node(nodes, cur, inv, oi, nx, ny, nz, h.t, stk, stride, sp, next, overflow);
+      visit_node(nodes, cur, inv, oi, nx, ny, nz, h.t, stk, stride, sp, next, ovf);
       // The two entries below the top are read unconditionally (a slot
       // below 0 reads slot 0 and is not used): with no child to go on with,
       // the walk pops the top; when the node it goes on with is the first
@@ -363,23 +370,24 @@ __device__ __forceinline__ bool trace_resume(const Node4* __restrict__ nodes, co
     // leaf loop
     while (leaf != SR_EMPTY) {
       visit_leaf(tris, leaf, o, d, h);
-      if (h.t > t_near && h.prim != INT32_MIN) {  // far hit: from now on only t <= t_near counts
-        far = true;
-        h.t = t_near_up;
-        h.prim = INT32_MIN;
-      }
-      leaf = SR_EMPTY;
-      if (cur != SR_EMPTY && cur < 0) {
-        leaf = cur;
-        const int top = stk_at(stk, stride, max(sp - 1, 0));
-        cur = sp > 0 ? top : SR_EMPTY;
-        sp = max(sp - 1, 0);
-      }
+      // a far hit: from now on only t <= t_near counts
+      const bool fh = h.t > t_near && h.prim != INT32_MIN;
+      far = far || fh;
+      h.t = fh ? t_near_up : h.t;
+      h.prim = fh ? INT32_MIN : h.prim;
+      // the node walk stopped at a second leaf: test it next and pop
+      // (SR_EMPTY is negative, so cur < 0 with cur != SR_EMPTY is a leaf)
+      const int top = stk_at(stk, stride, max(sp - 1, 0));
+      const bool again = cur != SR_EMPTY && cur < 0;
+      leaf = again ? cur : SR_EMPTY;
+      cur = again ? (sp > 0 ? top : SR_EMPTY) : cur;
+      sp = again ? max(sp - 1, 0) : sp;
     }
     walking = cur != SR_EMPTY || leaf != SR_EMPTY;
     n = __popcll(__ballot(walking));
     if (n < stop) break;
   }
+  if (ovf) atomicOr(overflow, 1u);
   if (live) {
     s.cur = cur;
     s.leaf = leaf;
