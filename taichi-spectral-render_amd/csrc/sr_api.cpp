@@ -734,7 +734,8 @@ struct LaunchBufs {
 };
 
 static int prepare_path_args(sr_handle* h, PathArgs& a, const LaunchBufs& lb, int width, int height, uint64_t seed,
-                             uint32_t stream, int first_block, int n_blocks, int max_depth, int flags = 0) {
+                             uint32_t stream, int first_block, int n_blocks, int max_depth, int flags = 0,
+                             bool adjoint = false) {
   hipStream_t st = h->st;
   fill_job(h, h->job_host, width, height, seed, stream, first_block, n_blocks, max_depth);
   h->job_host.sum = lb.sum;
@@ -753,6 +754,8 @@ static int prepare_path_args(sr_handle* h, PathArgs& a, const LaunchBufs& lb, in
   a.counters = w.counters;
   a.overflow = lb.overflow;
   a.chunk = 64;
+  // one job: its small tables go to LDS when they fit
+  a.tab = sr_lds_tables(h->n_mats, h->B, h->B * std::max(h->max_lights, 1), h->n_slots, adjoint);
   a.adj.rough_mat = a.adj.g_mat = a.adj.sig_mat = -2;
   a.adj.rough_idx = a.adj.g_idx = a.adj.sig_off = -1;
   h->last_work = lb.work;
@@ -880,7 +883,8 @@ int sr_render_adjoint(sr_handle* h, int width, int height, uint64_t seed, uint32
   if (n_blocks == 0) return 0;
   if ((rc = ensure_sum(h, width, height, h->st))) return rc;
   PathArgs a;
-  if ((rc = prepare_path_args(h, a, handle_bufs(h), width, height, seed, stream, first_block, n_blocks, max_depth)))
+  if ((rc = prepare_path_args(h, a, handle_bufs(h), width, height, seed, stream, first_block, n_blocks, max_depth, 0,
+                               true)))
     return rc;
   if ((rc = h->alb_off.upload(alb_off, (size_t)M, h->st))) return rc;
   a.adj.d_accum = dev_d_accum;

@@ -77,6 +77,21 @@ struct Job {
   int sss;           // 1: subsurface random walk (SR_RENDER_SUBSURFACE; the walk kernel only)
 };
 
+// Where a single-job launch keeps its scene's small tables -- the ones the
+// path loop reads at every segment: materials, per-band wavelengths, light
+// counts, areas and CDFs, light vertices, each material's EEM id -- in LDS,
+// after the traversal stacks.  Byte offsets from the region's start, all
+// multiples of 16; bytes = 0 leaves every table in global memory (a batch
+// of several jobs, or tables larger than the budget).
+struct LdsTables {
+  uint32_t bytes;
+  uint32_t mats, lights, lverts, lam_m, lam_nm, light_area, light_count, mat_eem;
+  uint32_t n_mats, n_lights, n_lverts;  // records copied: MatRec, LightRec, float4
+};
+// LDS the tables may take beside a 256-lane block's stacks (and the
+// adjoint's gradient table) so that three blocks still share a CU's 160 KB.
+#define SR_LDS_TABLE_BUDGET 13312
+
 struct PathArgs {
   const Job* jobs;                // [n_jobs] device table
   int n_jobs;
@@ -86,8 +101,14 @@ struct PathArgs {
   unsigned long long* counters;   // [n_jobs][4]: segments, traversals, non-finite drops
   uint32_t* overflow;             // traversal stack overflow flag
   uint32_t chunk;                 // items claimed per queue atomic (per wave)
+  LdsTables tab;                  // single job: its small tables in LDS (tab.bytes > 0)
   AdjMap adj;                     // adjoint only (single job)
 };
+
+// The LDS layout of a scene's small tables (n_lights = B * max(max_lights, 1)
+// light records, n_slots light slots of three vertices), or bytes = 0 when
+// they do not fit the budget left beside the adjoint's gradient table.
+LdsTables sr_lds_tables(int n_mats, int B, int n_lights, int n_slots, bool adjoint);
 
 struct ResolveParams {
   float cmf[SR_MAX_BANDS * 3];
@@ -100,7 +121,7 @@ int sr_launch_trace(const SceneDev& s, int n, const float* o, const float* d, fl
 int sr_path_grid(int* grid_out);
 int sr_adjoint_grid(int* grid_out);
 int sr_path_grid_bs(int block, int* grid_out);
-size_t sr_path_lds_bytes(int block, bool adjoint);
+size_t sr_path_lds_bytes(int block, bool adjoint, uint32_t table_bytes = 0);
 int sr_launch_path(const PathArgs& a, int grid, int block, hipStream_t st);
 int sr_launch_adjoint(const PathArgs& a, int grid, int block, hipStream_t st);
 int sr_path_sss_grid(int* grid_out);

@@ -22,6 +22,9 @@
 // forward path exactly.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstring>
+
 #include "sr_dual.h"
 #include "sr_kernels.h"
 
@@ -58,12 +61,68 @@ struct Lane {
   float Lw, Eb, sp_dr, stot_dsig, hg_sc;
 };
 
-// The lane's current job (wave-uniform) plus the adjoint configuration.
+// The small tables of the lane's scene that every segment reads (get_mat,
+// the band wavelengths, the NEE light records), seen where the launch keeps
+// them: in LDS for a single-job launch whose tables fit (copied there when
+// the kernel starts, so these reads wait for LDS instead of the caches),
+// otherwise in global memory.
+struct Tabs {
+  const MatRec* mats;      // [M][B]
+  const float* lam_m;      // [B]
+  const float* lam_nm;     // [B]
+  const float* light_area; // [B]
+  const int* light_count;  // [B]
+  const int* mat_eem;      // [M]
+  const LightRec* lights;  // [B][max_lights]
+  const float4* lverts;    // [slots][3]
+  int B;
+  __device__ __forceinline__ MatRec mat(int m, int b) const { return mats[m * B + b]; }  // get_mat, main.py:980-996
+};
+
+__device__ __forceinline__ Tabs global_tabs(const SceneDev& s) {
+  return Tabs{s.mats, s.lam_m, s.lam_nm, s.light_area, s.light_count, s.mat_eem, s.lights, s.lverts, s.B};
+}
+__device__ __forceinline__ Tabs lds_tabs(const SceneDev& s, const LdsTables& L, const char* base) {
+  return Tabs{reinterpret_cast<const MatRec*>(base + L.mats),       reinterpret_cast<const float*>(base + L.lam_m),
+              reinterpret_cast<const float*>(base + L.lam_nm),      reinterpret_cast<const float*>(base + L.light_area),
+              reinterpret_cast<const int*>(base + L.light_count),   reinterpret_cast<const int*>(base + L.mat_eem),
+              reinterpret_cast<const LightRec*>(base + L.lights),   reinterpret_cast<const float4*>(base + L.lverts),
+              s.B};
+}
+
+// The block's copy of job 0's tables (a single-job launch), made once by
+// all its threads before any path starts.
+__device__ __forceinline__ void copy_tables(const SceneDev& s, const LdsTables& L, char* base) {
+  const int t = threadIdx.x, n = blockDim.x;
+  auto copy16 = [&](uint32_t off, const void* src, uint32_t count) {
+    const float4* a = reinterpret_cast<const float4*>(src);
+    float4* d = reinterpret_cast<float4*>(base + off);
+    for (uint32_t i = t; i < count; i += n) d[i] = a[i];
+  };
+  auto copy4 = [&](uint32_t off, const void* src, uint32_t count) {
+    const int* a = reinterpret_cast<const int*>(src);
+    int* d = reinterpret_cast<int*>(base + off);
+    for (uint32_t i = t; i < count; i += n) d[i] = a[i];
+  };
+  copy16(L.mats, s.mats, L.n_mats * 2);
+  copy16(L.lights, s.lights, L.n_lights);
+  copy16(L.lverts, s.lverts, L.n_lverts);
+  copy4(L.lam_m, s.lam_m, s.B);
+  copy4(L.lam_nm, s.lam_nm, s.B);
+  copy4(L.light_area, s.light_area, s.B);
+  copy4(L.light_count, s.light_count, s.B);
+  copy4(L.mat_eem, s.mat_eem, s.M);
+  __syncthreads();
+}
+
+// The lane's current job (wave-uniform), its scene's tables, plus the
+// adjoint configuration.
 template <bool ADJ>
 struct Ctx {
   const Job& a;
   const AdjMap& adj;
   float* lds_grad;
+  Tabs t;
 };
 
 // The lane's uniform block (segment `seg`, slot `slot`); the key is the job's
@@ -167,10 +226,10 @@ __device__ __forceinline__ void start_path(Lane& L, const Ctx<ADJ>& c) {
 // both cosines are clamped to >= 1e-4 before NdotV * NdotL > 0 is tested).
 // With `walk` (SR_RENDER_SUBSURFACE) an opaque material with sigma_s > 0 has
 // no diffuse lobe here: that light enters the subsurface walk instead.
-__device__ __forceinline__ void evaluate_bsdf(const SceneDev& s, int vol_id, int surf_id, f3 V, f3 Lv, f3 N, int b,
+__device__ __forceinline__ void evaluate_bsdf(const Tabs& s, int vol_id, int surf_id, f3 V, f3 Lv, f3 N, int b,
                                               float& bsdf, float& pdf, bool walk = false) {
-  MatRec vm = get_mat(s, vol_id, b);
-  MatRec sm = get_mat(s, surf_id, b);
+  MatRec vm = s.mat(vol_id, b);
+  MatRec sm = s.mat(surf_id, b);
   float NdotL = tmax(1e-4f, dot(N, Lv));
   float NdotV = tmax(1e-4f, dot(N, V));
   bsdf = 0.0f;
@@ -198,10 +257,10 @@ __device__ __forceinline__ void evaluate_bsdf(const SceneDev& s, int vol_id, int
 // surface's albedo and, when it is the designated rough material, its
 // roughness.  The Fresnel term's dependence on k is not differentiated (k of
 // a diffuse albedo is ~1e-8 and moves F by less than 1e-7).
-__device__ __forceinline__ void evaluate_bsdf_d(const SceneDev& s, const AdjMap& m, int vol_id, int surf_id, f3 V,
+__device__ __forceinline__ void evaluate_bsdf_d(const Tabs& s, const AdjMap& m, int vol_id, int surf_id, f3 V,
                                                 f3 Lv, f3 N, int b, Dl& bsdf, Dl& pdf) {
-  MatRec vm = get_mat(s, vol_id, b);
-  MatRec sm = get_mat(s, surf_id, b);
+  MatRec vm = s.mat(vol_id, b);
+  MatRec sm = s.mat(surf_id, b);
   Dl r = dvar(sm.r, SL_ROUGH, surf_id == m.rough_mat);
   float NdotL = tmax(1e-4f, dot(N, Lv));
   float NdotV = tmax(1e-4f, dot(N, V));
@@ -253,8 +312,8 @@ __device__ __forceinline__ void scatter_surface(Lane& L, const Ctx<ADJ>& c, floa
     return;
   }
   const int ab = L.active;
-  MatRec sm = get_mat(s, L.emat, ab);
-  MatRec vm = get_mat(s, L.vol, ab);
+  MatRec sm = c.t.mat(L.emat, ab);
+  MatRec vm = c.t.mat(L.vol, ab);
   const f3 V = -L.dir;
   const f3 N = L.esn;
   const float ni = vm.n, ki = vm.k, nt = sm.n, kt = sm.k;
@@ -280,7 +339,7 @@ __device__ __forceinline__ void scatter_surface(Lane& L, const Ctx<ADJ>& c, floa
     if (sm.r < 0.008f) { est = F; pdf = 1e9f; }
     else branch = 1;
   } else {
-    float absorb = fdiv(4.0f * SR_PI * kt, s.lam_m[ab]);
+    float absorb = fdiv(4.0f * SR_PI * kt, c.t.lam_m[ab]);
     float dte = fdiv(-logf(-rs.u[3] + 1.0f + 1e-7f), sm.ss + absorb + 1e-6f);
     if (SSS && a.sss && !sm.tv && sm.ss > 0.0f) {
       // random-walk subsurface scattering (the TODO of main.py:1066) in
@@ -310,7 +369,7 @@ __device__ __forceinline__ void scatter_surface(Lane& L, const Ctx<ADJ>& c, floa
       float bsdf = (1.0f - F) * (albedo * SR_INV_PI) * oren_nayar(sm.r, NdotL, NdotV, S, V, N);
       pdf = 0.318309886184f * NdotL;
       on_NL = NdotL; on_NV = NdotV; on_albedo = albedo; on_S = S;
-      const int eid = s.mat_eem[L.emat];
+      const int eid = c.t.mat_eem[L.emat];
       if (eid > 0) {  // surface fluorescence reweights only (main.py:1090-1121)
         const float* row = s.eem + ((size_t)eid * s.B + ab) * s.B;
         float qy = 0.0f;
@@ -327,8 +386,8 @@ __device__ __forceinline__ void scatter_surface(Lane& L, const Ctx<ADJ>& c, floa
             cum += row[j] * rn;
             if (r4 < cum) { nw = j; break; }
           }
-          MatRec fm = get_mat(s, L.emat, nw);
-          float ab2 = fdiv(4.0f * SR_PI * fm.k, s.lam_m[nw]);
+          MatRec fm = c.t.mat(L.emat, nw);
+          float ab2 = fdiv(4.0f * SR_PI * fm.k, c.t.lam_m[nw]);
           bsdf *= expf(-ab2 * dte);
         }
       }
@@ -594,7 +653,7 @@ __device__ __forceinline__ void nee_setup(Lane& L, const Ctx<ADJ>& c, int surf, 
   const SceneDev& s = a.s;
   const int ab = L.active;
   L.nee_surf = surf;
-  const int nl = s.light_count[ab];
+  const int nl = c.t.light_count[ab];
   if (nl <= 0 || L.tp == 0.0f || shadow_dense(sigma_vol)) {
     L.fdirect = 0.0f;
     L.sdir = L.pos;
@@ -616,7 +675,7 @@ __device__ __forceinline__ void nee_setup(Lane& L, const Ctx<ADJ>& c, int surf, 
     lprim = s.lnodes[leaf].face;
     L.fdirect = s.shade[(size_t)lprim * 4].w / p;
   } else {
-    const LightRec* lr = s.lights + (size_t)ab * s.max_lights;
+    const LightRec* lr = c.t.lights + (size_t)ab * s.max_lights;
     float r = rn.u[0];
     // first i with r < cdf[i] (main.py:1310-1313) by binary search over the
     // non-decreasing CDF: the same light as the linear scan, O(log n) reads
@@ -631,9 +690,9 @@ __device__ __forceinline__ void nee_setup(Lane& L, const Ctx<ADJ>& c, int surf, 
     const LightRec le = lr[e];
     lslot = le.slot;
     lprim = le.prim;
-    L.fdirect = s.light_area[ab];
+    L.fdirect = c.t.light_area[ab];
   }
-  const float4* lv = s.lverts + (size_t)lslot * 3;
+  const float4* lv = c.t.lverts + (size_t)lslot * 3;
   f3 v0 = xyz(lv[0]), v1 = xyz(lv[1]), v2 = xyz(lv[2]);
   float bx = rn.u[1], by = rn.u[2];
   if (bx + by > 1.0f) { bx = 1.0f - bx; by = 1.0f - by; }
@@ -702,8 +761,8 @@ __device__ __forceinline__ float ext_flight(const Lane& L, const Ctx<ADJ>& c, fl
   const Job& a = c.a;
   const SceneDev& s = a.s;
   const int ab = L.active;
-  vm = get_mat(s, L.vol, ab);
-  sa = fdiv(4.0f * SR_PI * vm.k, s.lam_m[ab]);
+  vm = c.t.mat(L.vol, ab);
+  sa = fdiv(4.0f * SR_PI * vm.k, c.t.lam_m[ab]);
   re4 = draws(a, L, L.seg, SR_SLOT_EXT);
   return fdiv(-logf(-re4.u[0] + 1.0f - 1e-7f), vm.ss + sa + 1e-6f);
 }
@@ -764,7 +823,7 @@ __device__ __forceinline__ void on_extension(Lane& L, const Ctx<ADJ>& c, const H
     float st = fdiv(sa, vm.ss + sa + 1e-6f);
     if (re <= st) {
       absorb_branch = true;
-      const int eid = s.mat_eem[L.vol];
+      const int eid = c.t.mat_eem[L.vol];
       if (eid > 0) {
         const float* row = s.eem + ((size_t)eid * s.B + ab) * s.B;
         float qy = 0.0f;
@@ -785,7 +844,7 @@ __device__ __forceinline__ void on_extension(Lane& L, const Ctx<ADJ>& c, const H
           L.enew = nw;
           const U4 rx = draws(a, L, L.seg, SR_SLOT_AUX);
           L.edir = random_in_unit_sphere(rx.u[2], rx.u[3]);
-          em_new = get_mat(s, L.vol, nw).em;
+          em_new = c.t.mat(L.vol, nw).em;
           float em2 = fdiv((-(1.0f - T) + 1.0f) * em_new, sa + 1e-6f);
           est = (1.0f - T) + em2;
         } else {
@@ -812,7 +871,7 @@ __device__ __forceinline__ void on_extension(Lane& L, const Ctx<ADJ>& c, const H
       Dl Td = dexp((sad + vm.ss) * (-travel));
       Dl attd = Td + ((1.0f - Td) * vm.em) / (sad + 1e-6f);
       Dl estd;
-      const int eid = s.mat_eem[L.vol];
+      const int eid = c.t.mat_eem[L.vol];
       if (L.eaction == 1) estd = attd / (1.0f + 1e-6f);
       else if (L.eaction == 3) estd = (1.0f - Td) + (Td * em_new) / (sad + 1e-6f);
       else if (L.eaction == 2 || eid > 0) estd = attd;
@@ -841,11 +900,11 @@ __device__ __forceinline__ void on_extension(Lane& L, const Ctx<ADJ>& c, const H
   L.tp *= est;
   int surf = -1;
   if (L.eaction == 1) {
-    MatRec smt = get_mat(s, L.emat, L.band);
+    MatRec smt = c.t.mat(L.emat, L.band);
     if (smt.se) {  // direct hit on an emitter (main.py:1550-1559)
       float d2 = travel * travel;
       float cl = tmax(0.0f, dot(hn, -L.dir));
-      float lden = cl * s.light_area[L.active] + 1e-6f;
+      float lden = cl * c.t.light_area[L.active] + 1e-6f;
       if (a.nee_tree && cl > 0.0f && L.pl > 0) {
         // the light tree's pick probability at the previous NEE vertex; a
         // face the band's NEE cannot pick keeps the reference's weight
@@ -880,8 +939,8 @@ __device__ __forceinline__ void on_shadow(Lane& L, const Ctx<ADJ>& c, const HitD
   const SceneDev& s = a.s;
   L.n_trav++;
   const int ab = L.active;
-  MatRec vm = get_mat(s, L.svol, ab);
-  float absorb = fdiv(4.0f * SR_PI * vm.k, s.lam_m[ab]);
+  MatRec vm = c.t.mat(L.svol, ab);
+  float absorb = fdiv(4.0f * SR_PI * vm.k, c.t.lam_m[ab]);
   const bool sig_span = ADJ && L.svol == c.adj.sig_mat;
   L.stot *= expf(-(absorb + vm.ss) * h.t);
   if (sig_span) L.stot_dsig -= h.t;
@@ -892,9 +951,9 @@ __device__ __forceinline__ void on_shadow(Lane& L, const Ctx<ADJ>& c, const HitD
     done = true;
   } else {
     SurfHit sh = surface_of(s, h, L.sdir);
-    MatRec sm = get_mat(s, sh.mat, ab);
+    MatRec sm = c.t.mat(sh.mat, ab);
     float n_i = vm.n;
-    float n_t = sh.front ? sm.n : get_mat(s, 0, ab).n;
+    float n_t = sh.front ? sm.n : c.t.mat(0, ab).n;
     const float ff = (float)sh.front;
     f3 inc = sh.sn * ff - sh.sn * (1.0f - ff);
     f3 refr = refract(L.sdir, inc, n_i / n_t);
@@ -916,8 +975,8 @@ __device__ __forceinline__ void on_shadow(Lane& L, const Ctx<ADJ>& c, const HitD
         // step's exp(-sigma_t t) (t > EPSILON) and, when the light is
         // reached, exp(-sigma_t (smax - strav)) <= 1 leave stot <= 5e-7: no
         // contribution either way (main.py:1463-1469, 1329), so it is not traced.
-        const MatRec nv = get_mat(s, L.svol, ab);
-        if (shadow_dense(fdiv(4.0f * SR_PI * nv.k, s.lam_m[ab]) + nv.ss)) {
+        const MatRec nv = c.t.mat(L.svol, ab);
+        if (shadow_dense(fdiv(4.0f * SR_PI * nv.k, c.t.lam_m[ab]) + nv.ss)) {
           L.stot = 0.0f;
           done = true;
         }
@@ -941,14 +1000,14 @@ __device__ __forceinline__ void on_shadow(Lane& L, const Ctx<ADJ>& c, const HitD
       pdf = cosv * SR_INV_PI;
     } else if (L.nee_surf != -1) {
       cosv = tmax(0.0f, dot(L.esn, L.sdir));
-      evaluate_bsdf(s, L.vol, L.nee_surf, -L.dir, L.sdir, L.esn, ab, bsdf, pdf, SSS && c.a.sss);
+      evaluate_bsdf(c.t, L.vol, L.nee_surf, -L.dir, L.sdir, L.esn, ab, bsdf, pdf, SSS && c.a.sss);
     } else {
       cosv = tmax(0.0f, dot(L.dir, L.sdir));
-      bsdf = hg_eval(cosv, get_mat(s, L.vol, ab).g);
+      bsdf = hg_eval(cosv, c.t.mat(L.vol, ab).g);
       pdf = bsdf;
     }
     float factor = bsdf * cosv * L.stot * gt * aeff;
-    float emis = get_mat(s, lmat, L.band).em;
+    float emis = c.t.mat(lmat, L.band).em;
     float pdfl = fdiv(d2, tmax(1e-6f, cl) * aeff);
     float w = fdiv(pdfl * pdfl, pdfl * pdfl + pdf * pdf + 1e-6f);
     direct = L.tp * emis * factor * w;
@@ -956,9 +1015,9 @@ __device__ __forceinline__ void on_shadow(Lane& L, const Ctx<ADJ>& c, const HitD
       const AdjMap& m = c.adj;
       Dl bd, pd;
       if (L.nee_surf != -1) {
-        evaluate_bsdf_d(s, m, L.vol, L.nee_surf, -L.dir, L.sdir, L.esn, ab, bd, pd);
+        evaluate_bsdf_d(c.t, m, L.vol, L.nee_surf, -L.dir, L.sdir, L.esn, ab, bd, pd);
       } else {
-        bd = hg_eval_d(cosv, dvar(get_mat(s, L.vol, ab).g, SL_G, L.vol == m.g_mat));
+        bd = hg_eval_d(cosv, dvar(c.t.mat(L.vol, ab).g, SL_G, L.vol == m.g_mat));
         pd = bd;
       }
       Dl wd = dc(pdfl * pdfl) / ((pd * pd) + (pdfl * pdfl + 1e-6f));
@@ -1064,14 +1123,16 @@ __device__ __forceinline__ void ray_limits(const Lane& L, const Ctx<ADJ>& c, con
     // the adjoint's sigma_a material: its span derivative uses t.)  (k = 0
     // and sigma_s = 0 make that sum exactly 0.)
     const SceneDev& s = c.a.s;
-    const float4 mv = *reinterpret_cast<const float4*>(s.mats + L.svol * s.B + L.active);
+    const float4 mv = *reinterpret_cast<const float4*>(c.t.mats + L.svol * s.B + L.active);
     if (mv.y == 0.0f && mv.w == 0.0f && !(ADJ && L.svol == P.adj.sig_mat)) t_init = L.smax - L.strav;
   }
 }
 
-// REFILL picks the wave's traversal discipline (see the loop body).
-template <bool ADJ, bool SSS, bool REFILL>
-__device__ __forceinline__ void path_loop(const PathArgs& P, int* stk, float* lds_grad) {
+// REFILL picks the wave's traversal discipline (see the loop body); LT
+// reads the scene's small tables from their LDS copy at `tab` (a single-job
+// launch) instead of global memory.
+template <bool ADJ, bool SSS, bool REFILL, bool LT>
+__device__ __forceinline__ void path_loop(const PathArgs& P, int* stk, float* lds_grad, const char* tab) {
   const int stride = blockDim.x;
   const int lane = (int)__lane_id();
   // wave-uniform job cursor: a wave takes items from one job until that
@@ -1103,7 +1164,7 @@ __device__ __forceinline__ void path_loop(const PathArgs& P, int* stk, float* ld
   while (true) {
     cur = __builtin_amdgcn_readfirstlane(cur);
     const Job& J = *(const Job*)(jobs + cur);
-    const Ctx<ADJ> c{J, P.adj, lds_grad};
+    const Ctx<ADJ> c{J, P.adj, lds_grad, LT ? lds_tabs(J.s, P.tab, tab) : global_tabs(J.s)};
     if (L.phase == PH_NEW) {  // a path ended: the item's next band, if any
       L.band += 1;
       L.pass = 0;
@@ -1257,25 +1318,37 @@ __device__ __forceinline__ void path_loop(const PathArgs& P, int* stk, float* ld
 #define SR_WAVES_ADJ 3
 #endif
 
+// LDS of a launch: the lanes' traversal stacks (SR_STACK_SIZE entries per
+// lane, column-major), then the scene's small tables (tab.bytes, a single-job
+// launch), then the adjoint's gradient table.
+template <bool LT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void sr_path_kernel(PathArgs a) {
   extern __shared__ int lds_dyn[];
+  char* tab = reinterpret_cast<char*>(lds_dyn + SR_STACK_SIZE * blockDim.x);
   if (threadIdx.x == 0) atomicMin(a.clocks, wall_clock64());
-  path_loop<false, false, SR_REFILL_PATH>(a, lds_dyn + threadIdx.x, nullptr);
+  if (LT) copy_tables(a.jobs[0].s, a.tab, tab);
+  path_loop<false, false, SR_REFILL_PATH, LT>(a, lds_dyn + threadIdx.x, nullptr, tab);
 }
 
 // The same path loop with random-walk subsurface scattering (SR_RENDER_SUBSURFACE).
+template <bool LT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SR_WAVES_SSS, SR_WAVES_SSS))) void sr_path_sss_kernel(PathArgs a) {
   extern __shared__ int lds_dyn[];
+  char* tab = reinterpret_cast<char*>(lds_dyn + SR_STACK_SIZE * blockDim.x);
   if (threadIdx.x == 0) atomicMin(a.clocks, wall_clock64());
-  path_loop<false, true, SR_REFILL_SSS>(a, lds_dyn + threadIdx.x, nullptr);
+  if (LT) copy_tables(a.jobs[0].s, a.tab, tab);
+  path_loop<false, true, SR_REFILL_SSS, LT>(a, lds_dyn + threadIdx.x, nullptr, tab);
 }
 
+template <bool LT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SR_WAVES_ADJ, SR_WAVES_ADJ))) void sr_adjoint_kernel(PathArgs a) {
   extern __shared__ int lds_dyn[];
-  float* lds_grad = reinterpret_cast<float*>(lds_dyn + SR_STACK_SIZE * blockDim.x);
+  char* tab = reinterpret_cast<char*>(lds_dyn + SR_STACK_SIZE * blockDim.x);
+  float* lds_grad = reinterpret_cast<float*>(tab + (LT ? a.tab.bytes : 0u));
   for (int i = threadIdx.x; i < a.adj.n_params; i += blockDim.x) lds_grad[i] = 0.0f;
-  __syncthreads();
-  path_loop<true, false, SR_REFILL_ADJ>(a, lds_dyn + threadIdx.x, lds_grad);
+  if (LT) copy_tables(a.jobs[0].s, a.tab, tab);  // (ends with the barrier the gradient table needs)
+  else __syncthreads();
+  path_loop<true, false, SR_REFILL_ADJ, LT>(a, lds_dyn + threadIdx.x, lds_grad, tab);
   __syncthreads();
   for (int i = threadIdx.x; i < a.adj.n_params; i += blockDim.x) {
     float v = lds_grad[i];
@@ -1289,8 +1362,33 @@ __global__ void sr_fill_u32_kernel(uint32_t* __restrict__ p, size_t n, uint32_t 
   for (; i < n; i += step) p[i] = v;
 }
 
-size_t sr_path_lds_bytes(int block, bool adjoint) {
-  return (size_t)SR_STACK_SIZE * block * sizeof(int) + (adjoint ? SR_MAX_PARAMS * sizeof(float) : 0);
+size_t sr_path_lds_bytes(int block, bool adjoint, uint32_t table_bytes) {
+  return (size_t)SR_STACK_SIZE * block * sizeof(int) + table_bytes + (adjoint ? SR_MAX_PARAMS * sizeof(float) : 0);
+}
+
+LdsTables sr_lds_tables(int n_mats, int B, int n_lights, int n_slots, bool adjoint) {
+  LdsTables L;
+  std::memset(&L, 0, sizeof(L));
+  auto a16 = [](uint64_t x) { return (x + 15) & ~(uint64_t)15; };
+  uint64_t o = 0;
+  L.mats = (uint32_t)o;        o = a16(o + (uint64_t)n_mats * B * sizeof(MatRec));
+  L.lights = (uint32_t)o;      o = a16(o + (uint64_t)n_lights * sizeof(LightRec));
+  L.lverts = (uint32_t)o;      o = a16(o + (uint64_t)n_slots * 3 * sizeof(float4));
+  L.lam_m = (uint32_t)o;       o = a16(o + (uint64_t)B * 4);
+  L.lam_nm = (uint32_t)o;      o = a16(o + (uint64_t)B * 4);
+  L.light_area = (uint32_t)o;  o = a16(o + (uint64_t)B * 4);
+  L.light_count = (uint32_t)o; o = a16(o + (uint64_t)B * 4);
+  L.mat_eem = (uint32_t)o;     o = a16(o + (uint64_t)n_mats * 4);
+  const uint64_t budget = SR_LDS_TABLE_BUDGET - (adjoint ? SR_MAX_PARAMS * sizeof(float) : 0);
+  if (o > budget) {
+    std::memset(&L, 0, sizeof(L));
+    return L;
+  }
+  L.bytes = (uint32_t)o;
+  L.n_mats = (uint32_t)(n_mats * B);
+  L.n_lights = (uint32_t)n_lights;
+  L.n_lverts = (uint32_t)(n_slots * 3);
+  return L;
 }
 
 static int grid_for(const void* fn, int block, size_t lds, int* grid_out) {
@@ -1305,31 +1403,49 @@ static int grid_for(const void* fn, int block, size_t lds, int* grid_out) {
   return 0;
 }
 
+// A persistent grid that fits both instantiations of a kernel (tables in
+// global memory, or in LDS up to the whole table budget).
+static int grid_both(const void* fn_g, const void* fn_l, int block, bool adjoint, int* grid_out) {
+  const size_t table_max = SR_LDS_TABLE_BUDGET - (adjoint ? SR_MAX_PARAMS * sizeof(float) : 0);
+  int g0 = 0, g1 = 0;
+  if (grid_for(fn_g, block, sr_path_lds_bytes(block, adjoint), &g0)) return -1;
+  if (grid_for(fn_l, block, sr_path_lds_bytes(block, adjoint, (uint32_t)table_max), &g1)) return -1;
+  *grid_out = std::min(g0, g1);
+  return 0;
+}
+
 int sr_path_grid_bs(int block, int* grid_out) {
-  return grid_for((const void*)sr_path_kernel, block, sr_path_lds_bytes(block, false), grid_out);
+  return grid_both((const void*)sr_path_kernel<false>, (const void*)sr_path_kernel<true>, block, false, grid_out);
 }
 int sr_path_grid(int* grid_out) { return sr_path_grid_bs(256, grid_out); }
 int sr_adjoint_grid(int* grid_out) {
-  return grid_for((const void*)sr_adjoint_kernel, 256, sr_path_lds_bytes(256, true), grid_out);
+  return grid_both((const void*)sr_adjoint_kernel<false>, (const void*)sr_adjoint_kernel<true>, 256, true, grid_out);
 }
 
 int sr_launch_path(const PathArgs& a, int grid, int block, hipStream_t st) {
-  hipLaunchKernelGGL(sr_path_kernel, dim3(grid), dim3(block), sr_path_lds_bytes(block, false), st, a);
+  const size_t lds = sr_path_lds_bytes(block, false, a.tab.bytes);
+  if (a.tab.bytes) hipLaunchKernelGGL(sr_path_kernel<true>, dim3(grid), dim3(block), lds, st, a);
+  else hipLaunchKernelGGL(sr_path_kernel<false>, dim3(grid), dim3(block), lds, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int sr_path_sss_grid_bs(int block, int* grid_out) {
-  return grid_for((const void*)sr_path_sss_kernel, block, sr_path_lds_bytes(block, false), grid_out);
+  return grid_both((const void*)sr_path_sss_kernel<false>, (const void*)sr_path_sss_kernel<true>, block, false,
+                   grid_out);
 }
 int sr_path_sss_grid(int* grid_out) { return sr_path_sss_grid_bs(256, grid_out); }
 
 int sr_launch_path_sss(const PathArgs& a, int grid, int block, hipStream_t st) {
-  hipLaunchKernelGGL(sr_path_sss_kernel, dim3(grid), dim3(block), sr_path_lds_bytes(block, false), st, a);
+  const size_t lds = sr_path_lds_bytes(block, false, a.tab.bytes);
+  if (a.tab.bytes) hipLaunchKernelGGL(sr_path_sss_kernel<true>, dim3(grid), dim3(block), lds, st, a);
+  else hipLaunchKernelGGL(sr_path_sss_kernel<false>, dim3(grid), dim3(block), lds, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int sr_launch_adjoint(const PathArgs& a, int grid, int block, hipStream_t st) {
-  hipLaunchKernelGGL(sr_adjoint_kernel, dim3(grid), dim3(block), sr_path_lds_bytes(block, true), st, a);
+  const size_t lds = sr_path_lds_bytes(block, true, a.tab.bytes);
+  if (a.tab.bytes) hipLaunchKernelGGL(sr_adjoint_kernel<true>, dim3(grid), dim3(block), lds, st, a);
+  else hipLaunchKernelGGL(sr_adjoint_kernel<false>, dim3(grid), dim3(block), lds, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
