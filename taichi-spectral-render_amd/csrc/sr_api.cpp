@@ -207,6 +207,7 @@ struct sr_batch {
   };
   std::vector<Spec> specs;
   std::vector<Job> host_jobs;
+  uint32_t tab_bytes = 0;  // LDS table region of the launch (0: the jobs' tables stay in global memory)
   DevBuf jobs, work, overflow;
   int block = 256, grid_pct = 100;
   uint32_t chunk = 64;
@@ -741,6 +742,8 @@ static int prepare_path_args(sr_handle* h, PathArgs& a, const LaunchBufs& lb, in
   h->job_host.sum = lb.sum;
   h->job_host.nee_tree = ((flags & SR_RENDER_LIGHT_TREE) && h->n_lnodes > 0) ? 1 : 0;
   h->job_host.sss = (flags & SR_RENDER_SUBSURFACE) ? 1 : 0;
+  // the job's small tables go to LDS when they fit (single launches run 256-lane blocks)
+  h->job_host.tab = sr_lds_tables(h->n_mats, h->B, h->B * std::max(h->max_lights, 1), h->n_slots, 256, adjoint);
   HIPCHK(hipMemcpyAsync(lb.jobs, &h->job_host, sizeof(Job), hipMemcpyHostToDevice, st));
   if (reset_work(lb.work, 1, st)) return fail(SR_EHIP, "reset launch failed");
   if (sr_launch_fill_u32(lb.overflow, 1, 0u, st)) return fail(SR_EHIP, "reset launch failed");
@@ -754,8 +757,7 @@ static int prepare_path_args(sr_handle* h, PathArgs& a, const LaunchBufs& lb, in
   a.counters = w.counters;
   a.overflow = lb.overflow;
   a.chunk = 64;
-  // one job: its small tables go to LDS when they fit
-  a.tab = sr_lds_tables(h->n_mats, h->B, h->B * std::max(h->max_lights, 1), h->n_slots, adjoint);
+  a.tab_bytes = h->job_host.tab.bytes;
   a.adj.rough_mat = a.adj.g_mat = a.adj.sig_mat = -2;
   a.adj.rough_idx = a.adj.g_idx = a.adj.sig_off = -1;
   h->last_work = lb.work;
@@ -1108,6 +1110,7 @@ static int batch_enqueue(sr_batch* b, int grid) {
   a.counters = w.counters;
   a.overflow = (uint32_t*)b->overflow.p;
   a.chunk = b->chunk;
+  a.tab_bytes = b->tab_bytes;
   a.adj.rough_mat = a.adj.g_mat = a.adj.sig_mat = -2;
   a.adj.rough_idx = a.adj.g_idx = a.adj.sig_off = -1;
   const int lrc = batch_walks(b) ? sr_launch_path_sss(a, grid, b->block, st) : sr_launch_path(a, grid, b->block, st);
@@ -1143,6 +1146,20 @@ int sr_batch_launch(sr_batch* b, int flags) {
       fill_job(sp.h, b->host_jobs[j], sp.W, sp.H, sp.seed, sp.stream, sp.first, sp.nb, sp.depth);
       b->host_jobs[j].nee_tree = ((sp.flags & SR_RENDER_LIGHT_TREE) && sp.h->n_lnodes > 0) ? 1 : 0;
       b->host_jobs[j].sss = (sp.flags & SR_RENDER_SUBSURFACE) ? 1 : 0;
+      b->host_jobs[j].tab =
+          sr_lds_tables(sp.h->n_mats, sp.h->B, sp.h->B * std::max(sp.h->max_lights, 1), sp.h->n_slots, b->block, false);
+    }
+    // the jobs' tables share one LDS region per block (the block's waves
+    // switch jobs together): every job's must fit, or none goes to LDS
+    b->tab_bytes = 0;
+    bool all_fit = true;
+    for (const Job& jb : b->host_jobs) {
+      all_fit = all_fit && jb.tab.bytes > 0;
+      b->tab_bytes = std::max(b->tab_bytes, jb.tab.bytes);
+    }
+    if (!all_fit) {
+      b->tab_bytes = 0;
+      for (Job& jb : b->host_jobs) std::memset(&jb.tab, 0, sizeof(jb.tab));
     }
     int rc = b->jobs.upload(b->host_jobs.data(), (size_t)n, b->st);
     if (!rc) rc = b->work.ensure(work_bytes(n));

@@ -65,6 +65,22 @@ struct AdjMap {
   float scale;           // 1 / (first_block + n_blocks)
 };
 
+// Where a launch keeps a job's small scene tables -- the ones the path loop
+// reads at every segment: materials, per-band wavelengths, light counts,
+// areas and CDFs, light vertices, each material's EEM id -- in LDS, after
+// the traversal stacks.  Byte offsets from the region's start, all
+// multiples of 16; bytes = 0 leaves every table in global memory (tables
+// larger than the budget).
+struct LdsTables {
+  uint32_t bytes;
+  uint32_t mats, lights, lverts, lam_m, lam_nm, light_area, light_count, mat_eem;
+  uint32_t n_mats, n_lights, n_lverts;  // records copied: MatRec, LightRec, float4
+};
+// LDS the tables may take beside a 256-lane block's stacks so that three
+// blocks (12 waves) still share a CU's 160 KB; a smaller block gets its
+// share of it, and the adjoint's gradient table comes out of it.
+#define SR_LDS_TABLE_BUDGET 13312
+
 // One scene render: the unit of work of the persistent path kernel.  A
 // single render is a table of one job; a batch is a table of many.
 struct Job {
@@ -75,22 +91,9 @@ struct Job {
   uint32_t seed_lo, seed_hi, stream;
   int nee_tree;      // 1: NEE picks lights through the light tree (SR_RENDER_LIGHT_TREE)
   int sss;           // 1: subsurface random walk (SR_RENDER_SUBSURFACE; the walk kernel only)
+  LdsTables tab;     // this job's tables in the launch's LDS region (PathArgs::tab_bytes > 0)
 };
 
-// Where a single-job launch keeps its scene's small tables -- the ones the
-// path loop reads at every segment: materials, per-band wavelengths, light
-// counts, areas and CDFs, light vertices, each material's EEM id -- in LDS,
-// after the traversal stacks.  Byte offsets from the region's start, all
-// multiples of 16; bytes = 0 leaves every table in global memory (a batch
-// of several jobs, or tables larger than the budget).
-struct LdsTables {
-  uint32_t bytes;
-  uint32_t mats, lights, lverts, lam_m, lam_nm, light_area, light_count, mat_eem;
-  uint32_t n_mats, n_lights, n_lverts;  // records copied: MatRec, LightRec, float4
-};
-// LDS the tables may take beside a 256-lane block's stacks (and the
-// adjoint's gradient table) so that three blocks still share a CU's 160 KB.
-#define SR_LDS_TABLE_BUDGET 13312
 
 struct PathArgs {
   const Job* jobs;                // [n_jobs] device table
@@ -101,14 +104,15 @@ struct PathArgs {
   unsigned long long* counters;   // [n_jobs][4]: segments, traversals, non-finite drops
   uint32_t* overflow;             // traversal stack overflow flag
   uint32_t chunk;                 // items claimed per queue atomic (per wave)
-  LdsTables tab;                  // single job: its small tables in LDS (tab.bytes > 0)
+  uint32_t tab_bytes;             // LDS region for the jobs' tables (every job's tab.bytes fits); 0: global
   AdjMap adj;                     // adjoint only (single job)
 };
 
 // The LDS layout of a scene's small tables (n_lights = B * max(max_lights, 1)
 // light records, n_slots light slots of three vertices), or bytes = 0 when
-// they do not fit the budget left beside the adjoint's gradient table.
-LdsTables sr_lds_tables(int n_mats, int B, int n_lights, int n_slots, bool adjoint);
+// they do not fit the budget of a `block`-lane block.
+LdsTables sr_lds_tables(int n_mats, int B, int n_lights, int n_slots, int block, bool adjoint);
+size_t sr_lds_table_budget(int block, bool adjoint);
 
 struct ResolveParams {
   float cmf[SR_MAX_BANDS * 3];

@@ -1164,7 +1164,7 @@ __device__ __forceinline__ void path_loop(const PathArgs& P, int* stk, float* ld
   while (true) {
     cur = __builtin_amdgcn_readfirstlane(cur);
     const Job& J = *(const Job*)(jobs + cur);
-    const Ctx<ADJ> c{J, P.adj, lds_grad, LT ? lds_tabs(J.s, P.tab, tab) : global_tabs(J.s)};
+    const Ctx<ADJ> c{J, P.adj, lds_grad, LT ? lds_tabs(J.s, J.tab, tab) : global_tabs(J.s)};
     if (L.phase == PH_NEW) {  // a path ended: the item's next band, if any
       L.band += 1;
       L.pass = 0;
@@ -1228,6 +1228,14 @@ __device__ __forceinline__ void path_loop(const PathArgs& P, int* stk, float* ld
       flush_counters(P, cur, L, lane);
       if (cur + 1 >= P.n_jobs) break;
       cur += 1;
+      if (LT) {
+        // The block's waves move to the next job together: each arrives here
+        // once it is done with the last one, and the job's tables then
+        // replace the last job's in LDS (copy_tables ends with a barrier).
+        __syncthreads();
+        const Job& Jn = *(const Job*)(jobs + cur);
+        copy_tables(Jn.s, Jn.tab, const_cast<char*>(tab));
+      }
       gdry = false;
       lbase = lend = 0;
       continue;
@@ -1326,7 +1334,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
   extern __shared__ int lds_dyn[];
   char* tab = reinterpret_cast<char*>(lds_dyn + SR_STACK_SIZE * blockDim.x);
   if (threadIdx.x == 0) atomicMin(a.clocks, wall_clock64());
-  if (LT) copy_tables(a.jobs[0].s, a.tab, tab);
+  if (LT) copy_tables(a.jobs[0].s, a.jobs[0].tab, tab);
   path_loop<false, false, SR_REFILL_PATH, LT>(a, lds_dyn + threadIdx.x, nullptr, tab);
 }
 
@@ -1336,7 +1344,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SR_WAVES_SS
   extern __shared__ int lds_dyn[];
   char* tab = reinterpret_cast<char*>(lds_dyn + SR_STACK_SIZE * blockDim.x);
   if (threadIdx.x == 0) atomicMin(a.clocks, wall_clock64());
-  if (LT) copy_tables(a.jobs[0].s, a.tab, tab);
+  if (LT) copy_tables(a.jobs[0].s, a.jobs[0].tab, tab);
   path_loop<false, true, SR_REFILL_SSS, LT>(a, lds_dyn + threadIdx.x, nullptr, tab);
 }
 
@@ -1344,9 +1352,9 @@ template <bool LT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SR_WAVES_ADJ, SR_WAVES_ADJ))) void sr_adjoint_kernel(PathArgs a) {
   extern __shared__ int lds_dyn[];
   char* tab = reinterpret_cast<char*>(lds_dyn + SR_STACK_SIZE * blockDim.x);
-  float* lds_grad = reinterpret_cast<float*>(tab + (LT ? a.tab.bytes : 0u));
+  float* lds_grad = reinterpret_cast<float*>(tab + (LT ? a.tab_bytes : 0u));
   for (int i = threadIdx.x; i < a.adj.n_params; i += blockDim.x) lds_grad[i] = 0.0f;
-  if (LT) copy_tables(a.jobs[0].s, a.tab, tab);  // (ends with the barrier the gradient table needs)
+  if (LT) copy_tables(a.jobs[0].s, a.jobs[0].tab, tab);  // (ends with the barrier the gradient table needs)
   else __syncthreads();
   path_loop<true, false, SR_REFILL_ADJ, LT>(a, lds_dyn + threadIdx.x, lds_grad, tab);
   __syncthreads();
@@ -1366,7 +1374,13 @@ size_t sr_path_lds_bytes(int block, bool adjoint, uint32_t table_bytes) {
   return (size_t)SR_STACK_SIZE * block * sizeof(int) + table_bytes + (adjoint ? SR_MAX_PARAMS * sizeof(float) : 0);
 }
 
-LdsTables sr_lds_tables(int n_mats, int B, int n_lights, int n_slots, bool adjoint) {
+size_t sr_lds_table_budget(int block, bool adjoint) {
+  const size_t share = (size_t)SR_LDS_TABLE_BUDGET * (size_t)block / 256;
+  const size_t grads = adjoint ? SR_MAX_PARAMS * sizeof(float) : 0;
+  return share > grads ? share - grads : 0;
+}
+
+LdsTables sr_lds_tables(int n_mats, int B, int n_lights, int n_slots, int block, bool adjoint) {
   LdsTables L;
   std::memset(&L, 0, sizeof(L));
   auto a16 = [](uint64_t x) { return (x + 15) & ~(uint64_t)15; };
@@ -1379,8 +1393,7 @@ LdsTables sr_lds_tables(int n_mats, int B, int n_lights, int n_slots, bool adjoi
   L.light_area = (uint32_t)o;  o = a16(o + (uint64_t)B * 4);
   L.light_count = (uint32_t)o; o = a16(o + (uint64_t)B * 4);
   L.mat_eem = (uint32_t)o;     o = a16(o + (uint64_t)n_mats * 4);
-  const uint64_t budget = SR_LDS_TABLE_BUDGET - (adjoint ? SR_MAX_PARAMS * sizeof(float) : 0);
-  if (o > budget) {
+  if (o > sr_lds_table_budget(block, adjoint)) {
     std::memset(&L, 0, sizeof(L));
     return L;
   }
@@ -1406,7 +1419,7 @@ static int grid_for(const void* fn, int block, size_t lds, int* grid_out) {
 // A persistent grid that fits both instantiations of a kernel (tables in
 // global memory, or in LDS up to the whole table budget).
 static int grid_both(const void* fn_g, const void* fn_l, int block, bool adjoint, int* grid_out) {
-  const size_t table_max = SR_LDS_TABLE_BUDGET - (adjoint ? SR_MAX_PARAMS * sizeof(float) : 0);
+  const size_t table_max = sr_lds_table_budget(block, adjoint);
   int g0 = 0, g1 = 0;
   if (grid_for(fn_g, block, sr_path_lds_bytes(block, adjoint), &g0)) return -1;
   if (grid_for(fn_l, block, sr_path_lds_bytes(block, adjoint, (uint32_t)table_max), &g1)) return -1;
@@ -1423,8 +1436,8 @@ int sr_adjoint_grid(int* grid_out) {
 }
 
 int sr_launch_path(const PathArgs& a, int grid, int block, hipStream_t st) {
-  const size_t lds = sr_path_lds_bytes(block, false, a.tab.bytes);
-  if (a.tab.bytes) hipLaunchKernelGGL(sr_path_kernel<true>, dim3(grid), dim3(block), lds, st, a);
+  const size_t lds = sr_path_lds_bytes(block, false, a.tab_bytes);
+  if (a.tab_bytes) hipLaunchKernelGGL(sr_path_kernel<true>, dim3(grid), dim3(block), lds, st, a);
   else hipLaunchKernelGGL(sr_path_kernel<false>, dim3(grid), dim3(block), lds, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
@@ -1436,15 +1449,15 @@ int sr_path_sss_grid_bs(int block, int* grid_out) {
 int sr_path_sss_grid(int* grid_out) { return sr_path_sss_grid_bs(256, grid_out); }
 
 int sr_launch_path_sss(const PathArgs& a, int grid, int block, hipStream_t st) {
-  const size_t lds = sr_path_lds_bytes(block, false, a.tab.bytes);
-  if (a.tab.bytes) hipLaunchKernelGGL(sr_path_sss_kernel<true>, dim3(grid), dim3(block), lds, st, a);
+  const size_t lds = sr_path_lds_bytes(block, false, a.tab_bytes);
+  if (a.tab_bytes) hipLaunchKernelGGL(sr_path_sss_kernel<true>, dim3(grid), dim3(block), lds, st, a);
   else hipLaunchKernelGGL(sr_path_sss_kernel<false>, dim3(grid), dim3(block), lds, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int sr_launch_adjoint(const PathArgs& a, int grid, int block, hipStream_t st) {
-  const size_t lds = sr_path_lds_bytes(block, true, a.tab.bytes);
-  if (a.tab.bytes) hipLaunchKernelGGL(sr_adjoint_kernel<true>, dim3(grid), dim3(block), lds, st, a);
+  const size_t lds = sr_path_lds_bytes(block, true, a.tab_bytes);
+  if (a.tab_bytes) hipLaunchKernelGGL(sr_adjoint_kernel<true>, dim3(grid), dim3(block), lds, st, a);
   else hipLaunchKernelGGL(sr_adjoint_kernel<false>, dim3(grid), dim3(block), lds, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
