@@ -36,6 +36,7 @@ struct Lane {
   int phase, band, pi, pj, blk;
   f3 pos, dir;
   float tp, sp;
+  float dvol;  // the extension ray's free flight (scatter_volume), drawn before its trace
   int active, pl, vol, seg;
   float acc;
   int acc_band;
@@ -753,17 +754,21 @@ __device__ __forceinline__ void walk_step(Lane& L, const Ctx<ADJ>& c, const HitD
   L.phase = PH_EXT;
 }
 
-// Free-flight distance of scatter_volume (main.py:1203-1204), drawn from the
-// segment's EXT block before its trace: the trace then needs the exact
-// closest hit only up to dvol + EPSILON (trace_closest's t_near).
+// The lane's medium in its active band and its absorption 4 pi k / lambda.
 template <bool ADJ>
-__device__ __forceinline__ float ext_flight(const Lane& L, const Ctx<ADJ>& c, float& sa, MatRec& vm, U4& re4) {
-  const Job& a = c.a;
-  const SceneDev& s = a.s;
-  const int ab = L.active;
-  vm = c.t.mat(L.vol, ab);
-  sa = fdiv(4.0f * SR_PI * vm.k, c.t.lam_m[ab]);
-  re4 = draws(a, L, L.seg, SR_SLOT_EXT);
+__device__ __forceinline__ MatRec ext_medium(const Lane& L, const Ctx<ADJ>& c, float& sa) {
+  const MatRec vm = c.t.mat(L.vol, L.active);
+  sa = fdiv(4.0f * SR_PI * vm.k, c.t.lam_m[L.active]);
+  return vm;
+}
+// Free-flight distance of scatter_volume (main.py:1203-1204), drawn from the
+// segment's EXT block before its trace (kept in L.dvol): the trace then needs
+// the exact closest hit only up to dvol + EPSILON (trace_closest's t_near).
+template <bool ADJ>
+__device__ __forceinline__ float ext_flight(const Lane& L, const Ctx<ADJ>& c) {
+  float sa;
+  const MatRec vm = ext_medium<ADJ>(L, c, sa);
+  const U4 re4 = draws(c.a, L, L.seg, SR_SLOT_EXT);
   return fdiv(-logf(-re4.u[0] + 1.0f - 1e-7f), vm.ss + sa + 1e-6f);
 }
 
@@ -780,10 +785,9 @@ __device__ __forceinline__ void on_extension(Lane& L, const Ctx<ADJ>& c, const H
   L.n_seg++;
   L.n_trav++;
   const int ab = L.active;
-  MatRec vm;
   float sa;
-  U4 re4;
-  const float dvol = ext_flight<ADJ>(L, c, sa, vm, re4);
+  const MatRec vm = ext_medium<ADJ>(L, c, sa);
+  const float dvol = L.dvol;
   // h is exact for t <= dvol + EPSILON; beyond that only "some hit before
   // ESCAPE" (prim -2, t just above dvol + EPSILON) or "none" (t = ESCAPE)
   float ht = h.t;
@@ -819,6 +823,9 @@ __device__ __forceinline__ void on_extension(Lane& L, const Ctx<ADJ>& c, const H
     L.emat = hmat;
   } else if (travel >= 0.0f) {
     L.emat = -1;
+    // a volume event: the rest of the segment's EXT block (its first word
+    // drew dvol before the trace); surface hits and escapes never need it
+    const U4 re4 = draws(a, L, L.seg, SR_SLOT_EXT);
     float re = re4.u[1];
     float st = fdiv(sa, vm.ss + sa + 1e-6f);
     if (re <= st) {
@@ -1099,7 +1106,7 @@ __device__ __forceinline__ void flush_counters(const PathArgs& P, int j, Lane& L
 // surface lies before ESCAPE; shadow steps may start their search at the
 // light when the medium cannot attenuate.
 template <bool ADJ, bool SSS>
-__device__ __forceinline__ void ray_limits(const Lane& L, const Ctx<ADJ>& c, const PathArgs& P, float& t_init,
+__device__ __forceinline__ void ray_limits(Lane& L, const Ctx<ADJ>& c, const PathArgs& P, float& t_init,
                                            float& t_near) {
   t_init = 1e9f;
   t_near = INFINITY;
@@ -1109,11 +1116,9 @@ __device__ __forceinline__ void ray_limits(const Lane& L, const Ctx<ADJ>& c, con
     } else {
       // a path segment: hits at or beyond ESCAPE mean "escaped"
       // (main.py:1210), and past the free flight only their existence counts
-      MatRec vm;
-      float sa;
-      U4 re4;
       t_init = SR_ESCAPE;
-      t_near = ext_flight<ADJ>(L, c, sa, vm, re4) + SR_EPSILON;
+      L.dvol = ext_flight<ADJ>(L, c);
+      t_near = L.dvol + SR_EPSILON;
     }
   } else {
     // A shadow step through a medium with sigma_a + sigma_s = 0 needs only
@@ -1152,6 +1157,7 @@ __device__ __forceinline__ void path_loop(const PathArgs& P, int* stk, float* ld
   L.pass = 0;
   L.Lw = L.Eb = L.sp_dr = L.stot_dsig = L.hg_sc = 0.0f;
   L.fdirect = 0.0f;
+  L.dvol = 0.0f;
   bool has_item = false;
   int band_end = 0;  // the lane's item covers bands [.., band_end)
   int tr = 0;  // the lane's ray: 0 not started, 1 walking, 2 hit known

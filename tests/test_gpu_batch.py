@@ -51,6 +51,34 @@ def test_batch_matches_sequential(gpu):
         assert np.all(lat > 0) and lat.max() <= out["total_ms"] * 1.05 + 0.5
 
 
+def test_batch_tables_in_lds_and_in_global_memory_agree(gpu):
+    """256-lane blocks keep every job's small scene tables in LDS (the
+    block's waves switch jobs together); 64-lane blocks get a quarter of that
+    LDS share, too little for these scenes' tables, so the same batch reads
+    them from global memory.  Both give each scene the same image and the
+    same segment count."""
+    from spectral.batch import Batch
+
+    # the materials table alone ([M][B] records of 32 B) of the box scene
+    # outgrows a 64-lane block's 3,328-byte share of the 13,312-byte budget
+    c0, f0 = _scenes()[0]
+    assert f0.mat_f32.shape[0] * c0.bands * 32 > 13312 * 64 // 256
+    outs = []
+    for block in (256, 64):
+        bat = _renderers(gpu)
+        b = Batch()
+        b.set_config(block, 100, 64)
+        for c, r in bat:
+            r.clear_pixel_space()
+            b.submit(r, c.spp, c.depth, c.seed)
+        out = b.run(graph=False)
+        outs.append(([r.accum.cpu().numpy() for _, r in bat], list(out["segments"])))
+    (img_l, seg_l), (img_g, seg_g) = outs
+    assert seg_l == seg_g
+    for a, g in zip(img_l, img_g):
+        assert np.allclose(a, g, rtol=1e-5, atol=1e-6)
+
+
 def test_batch_config_and_tuner(gpu):
     from spectral.batch import Batch, tune
 
