@@ -64,8 +64,8 @@ struct Lane {
 
 // The small tables of the lane's scene that every segment reads (get_mat,
 // the band wavelengths, the NEE light records), seen where the launch keeps
-// them: in LDS for a single-job launch whose tables fit (copied there when
-// the kernel starts, so these reads wait for LDS instead of the caches),
+// them: in LDS when every job's tables fit the block's share (copied there
+// by the block, so these reads wait for LDS instead of the caches),
 // otherwise in global memory.
 struct Tabs {
   const MatRec* mats;      // [M][B]
@@ -91,8 +91,9 @@ __device__ __forceinline__ Tabs lds_tabs(const SceneDev& s, const LdsTables& L, 
               s.B};
 }
 
-// The block's copy of job 0's tables (a single-job launch), made once by
-// all its threads before any path starts.
+// The block's copy of a job's tables, made by all its threads when the
+// kernel starts (job 0) and each time the block's waves move to the next
+// job of a batch; ends with a barrier.
 __device__ __forceinline__ void copy_tables(const SceneDev& s, const LdsTables& L, char* base) {
   const int t = threadIdx.x, n = blockDim.x;
   auto copy16 = [&](uint32_t off, const void* src, uint32_t count) {
