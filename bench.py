@@ -461,6 +461,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="leave the batch64 key out of the default line")
+    ap.add_argument("--no-dead", action="store_true",
+                    help="leave out the end_dead_paths key (the same blocks with dead paths ended)")
     ap.add_argument("--no-inverse", action="store_true", help="leave the inverse_render key out")
     ap.add_argument("--spp-scale", type=int, default=8, help="batch64: divide each scene's spp by this")
     ap.add_argument("--build", choices=("host", "device"), default="host",
@@ -580,6 +582,23 @@ def main():
         "nonfinite": nonfinite,
         "roofline": roof,
     }
+    if not args.no_dead:
+        # the same blocks with SR_RENDER_END_DEAD_PATHS (a path ends once its
+        # throughput is exactly 0): the same image, fewer segments; the
+        # headline above counts the reference's segments
+        r.clear_pixel_space()
+        torch.cuda.synchronize()
+        t0 = time.time()
+        r.render(n_blocks=args.steps, max_depth=cfg.depth, seed=cfg.seed, end_dead_paths=True)
+        torch.cuda.synchronize()
+        wall_d = time.time() - t0
+        st_d = r.stats()
+        out["end_dead_paths"] = {"ms_per_step": round(wall_d * 1e3 / args.steps, 3),
+                                 "kernel_ms_per_step": round(r.last_kernel_ms() / args.steps, 3),
+                                 "segments_per_step": st_d["segments"] // max(args.steps, 1),
+                                 "Mseg_per_s": round(st_d["segments"] / wall_d / 1e6, 2),
+                                 "block_time_ratio": round(wall_d / wall, 4),
+                                 "nonfinite": st_d["nonfinite"]}
     del r
     if not args.no_adjoint:
         out["adjoint_forward_ratio"] = adjoint_ratio()

@@ -148,8 +148,8 @@ int sr_trace_closest(sr_handle* h, int n, const float* orig, const float* dir, f
  * Randomness: Philox-4x32-10 keyed by seed, counter (i*H + j, block,
  * band | stream << 8, call/4).  The work items of a call -- W * H *
  * n_blocks * ceil(B / 8), each up to 8 bands of one pixel and block -- must
- * number fewer than 2^31.  flags: 0 or SR_RENDER_LIGHT_TREE |
- * SR_RENDER_SUBSURFACE. */
+ * number fewer than 2^31.  flags: 0 or any of SR_RENDER_LIGHT_TREE,
+ * SR_RENDER_SUBSURFACE, SR_RENDER_END_DEAD_PATHS. */
 #define SR_RENDER_LIGHT_TREE 1 /* NEE picks its light through the light tree (importance by distance and
                                   orientation; the TODO of main.py:541) instead of the area CDF
                                   (main.py:1306-1313): same expected image, lower variance with many
@@ -159,6 +159,13 @@ int sr_trace_closest(sr_handle* h, int n, const float* orig, const float* dir, f
                                   entry and exit) instead of the single-scatter Oren-Nayar lobe: the TODO
                                   of main.py:1066.  Meshes must be closed (a walk that leaves an open
                                   mesh ends).  Not in the reference; off by default. */
+#define SR_RENDER_END_DEAD_PATHS 4 /* a path ends once its throughput is exactly 0 (a GGX reflection below
+                                      the horizon, a refraction into a conductor, a medium that absorbs
+                                      everything) instead of running to max_depth as cast_rays does
+                                      (main.py:1534): every later contribution of such a path is 0, so the
+                                      image is the same (the non-finite products 0 * inf the kernel
+                                      drops are not formed); fewer segments are executed and counted.
+                                      Off by default: sr_stats then counts the reference's segments. */
 int sr_render(sr_handle* h, int width, int height, uint64_t seed, uint32_t stream, int first_block, int n_blocks,
               int max_depth, int flags, float* dev_accum);
 

@@ -37,6 +37,9 @@ int fail(int code, const char* fmt, ...) {
     if (e_ != hipSuccess) return fail(SR_EHIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
   } while (0)
 
+// every SR_RENDER_* flag sr_render, sr_render_noalloc and sr_batch_submit accept
+#define SR_RENDER_FLAGS (SR_RENDER_LIGHT_TREE | SR_RENDER_SUBSURFACE | SR_RENDER_END_DEAD_PATHS)
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
@@ -742,6 +745,7 @@ static int prepare_path_args(sr_handle* h, PathArgs& a, const LaunchBufs& lb, in
   h->job_host.sum = lb.sum;
   h->job_host.nee_tree = ((flags & SR_RENDER_LIGHT_TREE) && h->n_lnodes > 0) ? 1 : 0;
   h->job_host.sss = (flags & SR_RENDER_SUBSURFACE) ? 1 : 0;
+  h->job_host.end_dead = (flags & SR_RENDER_END_DEAD_PATHS) ? 1 : 0;
   // the job's small tables go to LDS when they fit (single launches run 256-lane blocks)
   h->job_host.tab = sr_lds_tables(h->n_mats, h->B, h->B * std::max(h->max_lights, 1), h->n_slots, 256, adjoint);
   HIPCHK(hipMemcpyAsync(lb.jobs, &h->job_host, sizeof(Job), hipMemcpyHostToDevice, st));
@@ -797,7 +801,7 @@ static int render_with(sr_handle* h, const LaunchBufs& lb, int width, int height
 int sr_render(sr_handle* h, int width, int height, uint64_t seed, uint32_t stream, int first_block, int n_blocks,
               int max_depth, int flags, float* dev_accum) {
   g_err.clear();
-  if (flags & ~(SR_RENDER_LIGHT_TREE | SR_RENDER_SUBSURFACE)) return fail(SR_EINVAL, "unknown render flags 0x%x", flags);
+  if (flags & ~SR_RENDER_FLAGS) return fail(SR_EINVAL, "unknown render flags 0x%x", flags);
   int rc = check_render_args(h, width, height, first_block, n_blocks, max_depth, stream);
   if (rc) return rc;
   if (!dev_accum) return fail(SR_EINVAL, "dev_accum is NULL");
@@ -846,7 +850,7 @@ static int scratch_for_device(int device, unsigned char** out) {
 int sr_render_noalloc(sr_handle* h, int width, int height, uint64_t seed, uint32_t stream, int first_block,
                       int n_blocks, int max_depth, int flags, float* dev_accum) {
   g_err.clear();
-  if (flags & ~(SR_RENDER_LIGHT_TREE | SR_RENDER_SUBSURFACE)) return fail(SR_EINVAL, "unknown render flags 0x%x", flags);
+  if (flags & ~SR_RENDER_FLAGS) return fail(SR_EINVAL, "unknown render flags 0x%x", flags);
   int rc = check_render_args(h, width, height, first_block, n_blocks, max_depth, stream);
   if (rc) return rc;
   if (!dev_accum) return fail(SR_EINVAL, "dev_accum is NULL");
@@ -1056,7 +1060,7 @@ int sr_batch_submit(sr_batch* b, sr_handle* h, int width, int height, uint64_t s
                     int first_block, int n_blocks, int max_depth, int flags, float* dev_accum) {
   g_err.clear();
   if (!b) return fail(SR_EINVAL, "null batch");
-  if (flags & ~(SR_RENDER_LIGHT_TREE | SR_RENDER_SUBSURFACE)) return fail(SR_EINVAL, "unknown render flags 0x%x", flags);
+  if (flags & ~SR_RENDER_FLAGS) return fail(SR_EINVAL, "unknown render flags 0x%x", flags);
   int rc = check_render_args(h, width, height, first_block, n_blocks, max_depth, stream);
   if (rc) return rc;
   if (!dev_accum) return fail(SR_EINVAL, "dev_accum is NULL");
@@ -1146,6 +1150,7 @@ int sr_batch_launch(sr_batch* b, int flags) {
       fill_job(sp.h, b->host_jobs[j], sp.W, sp.H, sp.seed, sp.stream, sp.first, sp.nb, sp.depth);
       b->host_jobs[j].nee_tree = ((sp.flags & SR_RENDER_LIGHT_TREE) && sp.h->n_lnodes > 0) ? 1 : 0;
       b->host_jobs[j].sss = (sp.flags & SR_RENDER_SUBSURFACE) ? 1 : 0;
+      b->host_jobs[j].end_dead = (sp.flags & SR_RENDER_END_DEAD_PATHS) ? 1 : 0;
       b->host_jobs[j].tab =
           sr_lds_tables(sp.h->n_mats, sp.h->B, sp.h->B * std::max(sp.h->max_lights, 1), sp.h->n_slots, b->block, false);
     }

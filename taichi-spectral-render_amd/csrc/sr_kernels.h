@@ -91,6 +91,8 @@ struct Job {
   uint32_t seed_lo, seed_hi, stream;
   int nee_tree;      // 1: NEE picks lights through the light tree (SR_RENDER_LIGHT_TREE)
   int sss;           // 1: subsurface random walk (SR_RENDER_SUBSURFACE; the walk kernel only)
+  int end_dead;      // 1: a path ends once its throughput is exactly 0 (SR_RENDER_END_DEAD_PATHS)
+  int pad_;
   LdsTables tab;     // this job's tables in the launch's LDS region (PathArgs::tab_bytes > 0)
 };
 

@@ -510,6 +510,8 @@ __device__ __forceinline__ void finish_segment(Lane& L, const Ctx<ADJ>& c, float
   L.seg += 1;
   if (L.seg >= c.a.depth) {
     end_path<ADJ>(L, c);
+  } else if (c.a.end_dead && L.tp == 0.0f) {
+    end_path<ADJ>(L, c);  // every later contribution is 0 (SR_RENDER_END_DEAD_PATHS)
   } else if (!isfinite(L.tp)) {
     // non-finite throughput (e.g. the 0/0 rough BTDF estimator on leaving a
     // rough dielectric, main.py:1176-1185 with n_i == n_t): end the path

@@ -22,7 +22,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .render import Renderer
+from .render import Renderer, render_flags
 
 SR_BATCH_GRAPH = 1
 
@@ -52,13 +52,14 @@ class Batch:
         except Exception:
             pass
 
-    def submit(self, renderer: Renderer, n_blocks, max_depth, seed=0, stream=0, light_tree=False, subsurface=False):
+    def submit(self, renderer: Renderer, n_blocks, max_depth, seed=0, stream=0, light_tree=False, subsurface=False,
+               end_dead_paths=False):
         """Add a job: `n_blocks` spectral blocks of `renderer`'s scene into its
         accumulator (continuing from renderer.blocks_done), with the render
         modes of Renderer.render."""
         if renderer.accum is None:
             raise RuntimeError("renderer has no resolution")
-        flags = (1 if light_tree else 0) | (2 if subsurface else 0)
+        flags = render_flags(light_tree, subsurface, end_dead_paths)
         self._submit(renderer, int(n_blocks), int(max_depth), int(seed), int(stream), flags)
 
     def _submit(self, renderer, n_blocks, max_depth, seed, stream, flags):

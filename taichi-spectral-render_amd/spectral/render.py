@@ -21,6 +21,17 @@ import torch
 from . import _lib
 from .scene import FlatScene, flatten_scene, pixel_center_rays, scene_dimensions
 
+# SR_RENDER_* flags of include/spectral_render.h
+SR_RENDER_LIGHT_TREE = 1
+SR_RENDER_SUBSURFACE = 2
+SR_RENDER_END_DEAD_PATHS = 4
+
+
+def render_flags(light_tree=False, subsurface=False, end_dead_paths=False):
+    """The SR_RENDER_* mask of Renderer.render's mode arguments."""
+    return ((SR_RENDER_LIGHT_TREE if light_tree else 0) | (SR_RENDER_SUBSURFACE if subsurface else 0)
+            | (SR_RENDER_END_DEAD_PATHS if end_dead_paths else 0))
+
 
 class Renderer:
     """One scene, one accumulator.
@@ -146,7 +157,8 @@ class Renderer:
         self.flat.cam = cam
         self.clear_pixel_space()
 
-    def render(self, n_blocks=1, max_depth=7, seed=0, stream=0, light_tree=False, subsurface=False):
+    def render(self, n_blocks=1, max_depth=7, seed=0, stream=0, light_tree=False, subsurface=False,
+               end_dead_paths=False):
         """Run blocks [blocks_done, blocks_done + n_blocks) and fold them into
         the running spectral mean ``accum`` [W, H, B] (cast_rays x B +
         average_and_reset_block per block, main.py:1826-1836).
@@ -155,12 +167,14 @@ class Renderer:
         reference's NEE and BSDF strategies estimate the same integral; lower
         variance with many small lights).  ``subsurface``: opaque materials with a
         scattering coefficient scatter by a random walk inside their (closed)
-        mesh instead of the single-scatter Oren-Nayar lobe."""
+        mesh instead of the single-scatter Oren-Nayar lobe.  ``end_dead_paths``:
+        a path ends once its throughput is exactly 0 instead of running to
+        ``max_depth`` (the same image; fewer segments executed and counted)."""
         if self.accum is None:
             raise RuntimeError("set_resolution() first")
         self._bind_stream()
         _lib.call("sr_render", self._h, self.width, self.height, int(seed), int(stream), self.blocks_done,
-                  int(n_blocks), int(max_depth), (1 if light_tree else 0) | (2 if subsurface else 0),
+                  int(n_blocks), int(max_depth), render_flags(light_tree, subsurface, end_dead_paths),
                   _lib.ptr(self.accum))
         self.blocks_done += int(n_blocks)
         return self.accum

@@ -338,3 +338,34 @@ def test_many_lights_matched_seed_parity(gpu):
     assert ok.mean() >= 0.997, f"{(~ok).sum()} of {ok.size} pixel-bands beyond 3 sigma"
     lit = mc > 1e-3
     assert (diff[lit] / mc[lit]).mean() < 0.01
+
+
+@pytest.mark.parametrize("make", [
+    lambda: configs.box_icosphere(width=32, height=32, spp=4),
+    lambda: configs.heightfield_1m(width=24, height=16, spp=3, nx=64, nz=32),
+    lambda: configs.volume_fluor(width=16, height=16, spp=2),
+], ids=["box_icosphere", "heightfield", "volume_fluor"])
+def test_end_dead_paths_keeps_the_image(gpu, make):
+    """SR_RENDER_END_DEAD_PATHS ends a path once its throughput is exactly 0:
+    the same image (each path's later contributions are all 0), fewer
+    segments, and none of the non-finite 0 * inf products the kernel drops;
+    alone and as a batch job."""
+    from spectral.batch import Batch
+
+    cfg = make()
+    flat = flatten_scene(cfg.scene, cfg.bands, cfg.wl_min, cfg.wl_max)
+    full = _renderer(flat, cfg, cfg.width, cfg.height, gpu)
+    ref = full.render(n_blocks=cfg.spp, max_depth=cfg.depth, seed=cfg.seed).cpu().numpy()
+    st_ref = full.stats()
+    cut = _renderer(flat, cfg, cfg.width, cfg.height, gpu)
+    got = cut.render(n_blocks=cfg.spp, max_depth=cfg.depth, seed=cfg.seed, end_dead_paths=True).cpu().numpy()
+    st = cut.stats()
+    assert np.allclose(got, ref, rtol=1e-5, atol=1e-7)
+    assert 0 < st["segments"] < st_ref["segments"]
+    assert st["nonfinite"] <= st_ref["nonfinite"]
+    bat = _renderer(flat, cfg, cfg.width, cfg.height, gpu)
+    b = Batch()
+    b.submit(bat, cfg.spp, cfg.depth, cfg.seed, end_dead_paths=True)
+    out = b.run(graph=False)
+    assert np.allclose(bat.accum.cpu().numpy(), got, rtol=1e-5, atol=1e-7)
+    assert int(out["segments"][0]) == st["segments"]
