@@ -108,17 +108,23 @@ def adjoint_ratio(reps=2):
     ds = DifferentiableScene(flat, inverse_volume_spec(), cfg.width, cfg.height, cfg.spp, cfg.depth, cfg.seed,
                              device="cuda:0")
     d_acc = torch.rand((cfg.width, cfg.height, cfg.bands), device="cuda:0")
-    fwd, adj = [], []
+    fwd, adj, full = [], [], []
     for _ in range(reps + 1):
-        ds.render()
+        ds.render()  # the differentiable forward: dead paths ended, as the adjoint replays them
         fwd.append(ds.r.last_kernel_ms())
         segs = ds.r.stats()["segments"]
         ds.adjoint(d_acc)
         adj.append(ds.r.last_kernel_ms())
-    fwd, adj = min(fwd[1:]), min(adj[1:])
+        ds.r.clear_pixel_space()  # the same image with every path run to the depth limit
+        ds.r.render(n_blocks=ds.spp, max_depth=ds.depth, seed=ds.seed)
+        full.append(ds.r.last_kernel_ms())
+    fwd, adj, full = min(fwd[1:]), min(adj[1:]), min(full[1:])
     return {"value": round(adj / fwd, 3), "forward_ms": round(fwd, 2), "adjoint_ms": round(adj, 2),
+            "adjoint_over_full_forward": round(adj / full, 3), "full_forward_ms": round(full, 2),
             "workload": f"{cfg.name}: {flat.n_tris} tris, {cfg.bands} bands, {cfg.width}x{cfg.height}, "
-                        f"{cfg.spp} spp, depth {cfg.depth}, {ds.n_params} params ({segs} segments per forward)"}
+                        f"{cfg.spp} spp, depth {cfg.depth}, {ds.n_params} params ({segs} segments per forward; "
+                        f"forward and adjoint end paths whose throughput is 0, full_forward runs every path "
+                        f"to the depth limit)"}
 
 
 def bench_inverse(iterations=3, width=256, height=256, spp=64, log=print):

@@ -889,8 +889,12 @@ int sr_render_adjoint(sr_handle* h, int width, int height, uint64_t seed, uint32
   if (n_blocks == 0) return 0;
   if ((rc = ensure_sum(h, width, height, h->st))) return rc;
   PathArgs a;
-  if ((rc = prepare_path_args(h, a, handle_bufs(h), width, height, seed, stream, first_block, n_blocks, max_depth, 0,
-                               true)))
+  // Paths are replayed only until their throughput is exactly 0: every
+  // gradient term after that point is R x (...) or a contribution's own
+  // derivative, and R (the radiance still to come) and the contributions are
+  // exactly 0 there, so the gradient is the same whether or not they run.
+  if ((rc = prepare_path_args(h, a, handle_bufs(h), width, height, seed, stream, first_block, n_blocks, max_depth,
+                               SR_RENDER_END_DEAD_PATHS, true)))
     return rc;
   if ((rc = h->alb_off.upload(alb_off, (size_t)M, h->st))) return rc;
   a.adj.d_accum = dev_d_accum;

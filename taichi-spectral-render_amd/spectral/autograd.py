@@ -106,9 +106,11 @@ class DifferentiableScene:
         self.current_params = th.copy()
 
     def render(self, stream=0):
-        """Fresh render of `spp` blocks; returns the spectral mean [W, H, B]."""
+        """Fresh render of `spp` blocks; returns the spectral mean [W, H, B].
+        Paths end once their throughput is exactly 0 (the same image; the
+        adjoint replays them the same way)."""
         self.r.clear_pixel_space()
-        self.r.render(n_blocks=self.spp, max_depth=self.depth, seed=self.seed, stream=stream)
+        self.r.render(n_blocks=self.spp, max_depth=self.depth, seed=self.seed, stream=stream, end_dead_paths=True)
         return self.r.accum
 
     def adjoint(self, d_accum, stream=0, flags=0):
