@@ -290,5 +290,5 @@ def test_tree_layout_and_bounds(gpu):
 def test_unknown_render_flag_rejected(gpu):
     cfg, flat = single_light_scene()
     r = Renderer(flat, cfg.bands, cfg.wl_min, cfg.wl_max, width=8, height=8, device=gpu)
-    with pytest.raises(_lib.SpectralRenderError, match="flags"):
-        _lib.call("sr_render", r._h, 8, 8, 0, 0, 0, 1, 4, 4, _lib.ptr(r.accum))
+    with pytest.raises(_lib.SpectralRenderError, match="flags"):  # 8: past SR_RENDER_END_DEAD_PATHS (4)
+        _lib.call("sr_render", r._h, 8, 8, 0, 0, 0, 1, 4, 8, _lib.ptr(r.accum))
