@@ -96,7 +96,6 @@ struct Job {
   LdsTables tab;     // this job's tables in the launch's LDS region (PathArgs::tab_bytes > 0)
 };
 
-
 struct PathArgs {
   const Job* jobs;                // [n_jobs] device table
   int n_jobs;
