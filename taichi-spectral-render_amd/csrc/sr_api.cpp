@@ -398,6 +398,8 @@ int sr_upload_scene(sr_handle* h, const float* tris_v, const float* tris_n, cons
     }
     max_lights = std::max(max_lights, (int)per_band[b].size());
   }
+  // (the path kernels copy these records to LDS as one 16-byte unit each)
+  static_assert(sizeof(LightRec) == 16, "LightRec is 16 bytes");
   std::vector<LightRec> lr((size_t)B * std::max(max_lights, 1));
   std::vector<int> lcount((size_t)B, 0);
   std::vector<float> larea((size_t)B, 0.0f);
