@@ -221,9 +221,10 @@ struct sr_batch {
   double clock_khz = 100000.0;
 };
 
-// Work area of a launch over n jobs: queues[n], done[n] (u32), then
-// clocks[n + 1] and counters[4 n] (u64, 8-byte aligned).
-static size_t work_bytes(int n) { return ((size_t)8 * n + 7) / 8 * 8 + 8 * (size_t)(n + 1) + 32 * (size_t)n; }
+// Work area of a launch over n jobs: queues[n][SR_QUEUES], done[n] (u32),
+// then clocks[n + 1] and counters[4 n] (u64, 8-byte aligned).
+static constexpr size_t work_u32_bytes(int n) { return ((size_t)4 * (SR_QUEUES + 1) * n + 7) / 8 * 8; }
+static constexpr size_t work_bytes(int n) { return work_u32_bytes(n) + 8 * (size_t)(n + 1) + 32 * (size_t)n; }
 struct WorkView {
   uint32_t* queues;
   uint32_t* done;
@@ -233,13 +234,14 @@ struct WorkView {
 static WorkView work_view(void* base, int n) {
   WorkView w;
   w.queues = (uint32_t*)base;
-  w.done = w.queues + n;
-  w.clocks = (unsigned long long*)((char*)base + ((size_t)8 * n + 7) / 8 * 8);
+  w.done = w.queues + (size_t)SR_QUEUES * n;
+  w.clocks = (unsigned long long*)((char*)base + work_u32_bytes(n));
   w.counters = w.clocks + (n + 1);
   return w;
 }
 // A job's work queue is a u32 that every wave advances by `chunk` per claim,
-// with at most one claim per wave past the end: the counter must not wrap.
+// with at most one claim per wave past the end: the counter must not wrap
+// (each of a job's queues holds at most its n_items).
 static bool queue_overflows(uint32_t n_items, int grid, int block, uint32_t chunk) {
   const unsigned long long waves = ((unsigned long long)grid * (unsigned long long)block + 63) / 64;
   return (unsigned long long)n_items + (waves + 1) * (unsigned long long)chunk >= (1ull << 32);
@@ -248,7 +250,7 @@ static bool queue_overflows(uint32_t n_items, int grid, int block, uint32_t chun
 // Graph-safe reset of the work area (kernels only, no memset nodes).
 static int reset_work(void* base, int n, hipStream_t st) {
   WorkView w = work_view(base, n);
-  if (sr_launch_fill_u32(w.queues, (size_t)2 * n, 0u, st)) return -1;
+  if (sr_launch_fill_u32(w.queues, (size_t)(SR_QUEUES + 1) * n, 0u, st)) return -1;
   if (sr_launch_fill_u32((uint32_t*)w.clocks, (size_t)2 * (n + 1), 0xFFFFFFFFu, st)) return -1;
   if (sr_launch_fill_u32((uint32_t*)w.counters, (size_t)8 * n, 0u, st)) return -1;
   return 0;
@@ -714,6 +716,14 @@ static void fill_job(sr_handle* h, Job& j, int width, int height, uint64_t seed,
   j.first_block = first_block;
   j.n_blocks = n_blocks;
   j.tiled = (width % 8 == 0 && height % 8 == 0) ? 1 : 0;
+  // the XCD queues split each spectral block's items in eighths, on tile-group bounds when tiled
+  j.per_block = (uint32_t)((unsigned long long)width * height * sr_items_per_pixel(h->B));
+  const uint64_t unit = j.tiled ? 64 : 1, units = j.per_block / unit;
+  for (int x = 0; x < SR_QUEUES; ++x) {
+    const uint64_t lo = (uint64_t)x * units / SR_QUEUES * unit, hi = (uint64_t)(x + 1) * units / SR_QUEUES * unit;
+    j.q_off[x] = (uint32_t)lo;
+    j.q_len[x] = (uint32_t)(hi - lo);
+  }
   j.seed_lo = (uint32_t)seed;
   j.seed_hi = (uint32_t)(seed >> 32);
   j.stream = stream;
@@ -825,6 +835,7 @@ constexpr size_t kScratchSumBytes = (size_t)SR_NOALLOC_MAX_ELEMS * sizeof(float)
 constexpr size_t kScratchJobOff = kScratchSumBytes;
 constexpr size_t kScratchWorkOff = kScratchJobOff + (sizeof(Job) + 255) / 256 * 256;
 constexpr size_t kScratchOvfOff = kScratchWorkOff + 256;
+static_assert(work_bytes(1) <= 256, "a one-job work area fits its scratch slot");
 constexpr size_t kScratchBytes = kScratchOvfOff + 256;
 static_assert(sizeof(Job) % 8 == 0, "the job table is copied whole");
 constexpr int kMaxDevices = 64;

@@ -81,25 +81,35 @@ struct LdsTables {
 // share of it, and the adjoint's gradient table comes out of it.
 #define SR_LDS_TABLE_BUDGET 13312
 
+// Work queues per job, one per XCD.  The blocks of a launch are dealt to
+// the 8 XCDs in turn (block b runs on XCD b % 8) and each XCD has its own
+// L2: queue x holds the x-th eighth of every spectral block's tiles, so the
+// waves of one XCD work on one screen region at a time.  A wave drains its
+// XCD's queue first, then the others in turn.
+#define SR_QUEUES 8
+
 // One scene render: the unit of work of the persistent path kernel.  A
 // single render is a table of one job; a batch is a table of many.
 struct Job {
   SceneDev s;
   float* sum;        // [W][H][B] block sums of this launch (zero on entry)
-  uint32_t n_items;  // W * H * n_blocks
+  uint32_t n_items;  // W * H * n_blocks * items per pixel
   int W, H, depth, first_block, n_blocks, tiled;
   uint32_t seed_lo, seed_hi, stream;
   int nee_tree;      // 1: NEE picks lights through the light tree (SR_RENDER_LIGHT_TREE)
   int sss;           // 1: subsurface random walk (SR_RENDER_SUBSURFACE; the walk kernel only)
   int end_dead;      // 1: a path ends once its throughput is exactly 0 (SR_RENDER_END_DEAD_PATHS)
-  int pad_;
+  uint32_t per_block;  // items per spectral block: W * H * items per pixel
+  // queue x: items [q_off[x], q_off[x] + q_len[x]) of every spectral block
+  // (whole 64-item tile groups when tiled), q_len[x] * n_blocks in all
+  uint32_t q_off[SR_QUEUES], q_len[SR_QUEUES];
   LdsTables tab;     // this job's tables in the launch's LDS region (PathArgs::tab_bytes > 0)
 };
 
 struct PathArgs {
   const Job* jobs;                // [n_jobs] device table
   int n_jobs;
-  uint32_t* queues;               // [n_jobs] next work item (zero on entry)
+  uint32_t* queues;               // [n_jobs][SR_QUEUES] next item of each queue (zero on entry)
   uint32_t* done_items;           // [n_jobs] completed items (zero on entry)
   unsigned long long* clocks;     // [n_jobs + 1] wall clock: [0] launch start (min), [1 + j] job j done
   unsigned long long* counters;   // [n_jobs][4]: segments, traversals, non-finite drops

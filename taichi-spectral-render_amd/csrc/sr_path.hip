@@ -1088,6 +1088,14 @@ __device__ __forceinline__ void item_to_path(const Job& a, uint32_t item, int& p
   band_end = min(band + SR_BANDS_PER_ITEM, a.s.B);
 }
 
+// Item k of job J's queue q (k < q_len[q] * n_blocks): the queue's share of
+// every spectral block, block after block.
+__device__ __forceinline__ uint32_t queue_item(const Job& J, int q, uint32_t k) {
+  const uint32_t len = J.q_len[q];
+  const uint32_t b = k / len;
+  return b * J.per_block + J.q_off[q] + (k - b * len);
+}
+
 // Per-wave flush of the lane counters into job j's slot.
 __device__ __forceinline__ void flush_counters(const PathArgs& P, int j, Lane& L, int lane) {
   unsigned long long seg = L.n_seg, trav = L.n_trav, nf = L.n_nf;
@@ -1144,11 +1152,16 @@ __device__ __forceinline__ void path_loop(const PathArgs& P, int* stk, float* ld
   const int stride = blockDim.x;
   const int lane = (int)__lane_id();
   // wave-uniform job cursor: a wave takes items from one job until that
-  // job's queue is dry and none of its lanes still runs a path of it.
-  // Items come in chunks of P.chunk per atomic (a wave-local range).
+  // job's queues are dry and none of its lanes still runs a path of it.
+  // Items come in chunks of P.chunk per atomic (a wave-local range
+  // [lbase, lend) of queue lq).  The wave claims from its XCD's queue
+  // first (q0: blocks are dealt to the XCDs in turn), then, each time the
+  // queue it claims from runs dry, from the next one; gdry once all are.
+  const int q0 = (int)(blockIdx.x % SR_QUEUES);
   int cur = 0;
   int ext_streak = 0;
   bool gdry = false;
+  int qn = 0, lq = 0;
   uint32_t lbase = 0, lend = 0;
   Lane L;
   L.phase = PH_PARK;  // waiting for a work item
@@ -1201,20 +1214,25 @@ __device__ __forceinline__ void path_loop(const PathArgs& P, int* stk, float* ld
       const uint32_t n = (uint32_t)__popcll(mask);
       const uint32_t left = lend - lbase;
       uint32_t nb = 0, nend = 0;
+      const int nq = (q0 + qn) % SR_QUEUES;
       if (left < n && !gdry) {
+        const uint32_t qitems = J.q_len[nq] * (uint32_t)J.n_blocks;
         const int leader = __ffsll((long long)mask) - 1;
         uint32_t got = 0;
-        if (lane == leader) got = atomicAdd(P.queues + cur, P.chunk);
+        if (lane == leader) got = atomicAdd(P.queues + cur * SR_QUEUES + nq, P.chunk);
         nb = __builtin_amdgcn_readlane(got, leader);
-        nend = nb >= J.n_items ? nb : min(nb + P.chunk, J.n_items);
-        if (nb + P.chunk >= J.n_items) gdry = true;
+        nend = nb >= qitems ? nb : min(nb + P.chunk, qitems);
+        if (nb + P.chunk >= qitems) {
+          qn += 1;
+          gdry = qn >= SR_QUEUES;
+        }
       }
       if (need) {
         const uint32_t r =
             __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
         uint32_t item = 0xFFFFFFFFu;
-        if (r < left) item = lbase + r;
-        else if (nb + (r - left) < nend) item = nb + (r - left);
+        if (r < left) item = queue_item(J, lq, lbase + r);
+        else if (nb + (r - left) < nend) item = queue_item(J, nq, nb + (r - left));
         if (item != 0xFFFFFFFFu) {
           item_to_path(J, item, L.pi, L.pj, L.blk, L.band, band_end);
           L.pass = 0;
@@ -1227,6 +1245,7 @@ __device__ __forceinline__ void path_loop(const PathArgs& P, int* stk, float* ld
       } else {
         lbase = min(nb + (n - left), nend);
         lend = nend;
+        lq = nq;
       }
     }
     if (L.phase == PH_NEXT || L.phase == PH_REPLAY) start_path<ADJ>(L, c);
@@ -1246,6 +1265,7 @@ __device__ __forceinline__ void path_loop(const PathArgs& P, int* stk, float* ld
         copy_tables(Jn.s, Jn.tab, const_cast<char*>(tab));
       }
       gdry = false;
+      qn = lq = 0;
       lbase = lend = 0;
       continue;
     }

@@ -1,7 +1,8 @@
 """Edge cases of the render entry point against the oracle: a single pixel,
 a non-square image whose sides are not multiples of the 8x8 item tiles (the
-untiled item mapping) and one that is (the tiled mapping), path depth 0
-and 1, the largest band count, zero blocks, and run-to-run determinism."""
+untiled item mapping) and one that is (the tiled mapping), every path of a
+launch rendered exactly once whatever the image shape, path depth 0 and 1,
+the largest band count, zero blocks, and run-to-run determinism."""
 import numpy as np
 import pytest
 
@@ -38,6 +39,18 @@ def test_shapes_matched_seed(gpu, W, H):
     assert (~close).sum() <= max(1, int(0.03 * close.size)), (~close).sum()
     assert g.sum() == pytest.approx(cpu.sum(), rel=0.03)
     assert abs(st["segments"] - st0["segments"]) <= 0.01 * st0["segments"] + 5
+
+
+@pytest.mark.parametrize("W,H,blocks", [(1, 1, 3), (3, 5, 1), (8, 8, 1), (24, 40, 3), (64, 64, 2), (200, 8, 1)])
+def test_every_path_rendered_once(gpu, W, H, blocks):
+    # At depth 1 a path is exactly one segment, so the launch's segment count
+    # is the number of (pixel, block, band) paths the work queues handed out:
+    # one queue per XCD, each with its share of every block (empty shares
+    # for images of fewer tile groups than queues), drained by every wave.
+    cfg = configs.box_icosphere()
+    _, r, _ = _setup(cfg, gpu, W, H)
+    r.render(n_blocks=blocks, max_depth=1, seed=0)
+    assert r.stats()["segments"] == W * H * blocks * cfg.bands
 
 
 def test_depth_zero_and_one(gpu):
