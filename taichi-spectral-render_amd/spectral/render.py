@@ -14,6 +14,8 @@ only the allocator and stream provider here.
 from __future__ import annotations
 
 import ctypes
+import hashlib
+import json
 
 import numpy as np
 import torch
@@ -31,6 +33,28 @@ def render_flags(light_tree=False, subsurface=False, end_dead_paths=False):
     """The SR_RENDER_* mask of Renderer.render's mode arguments."""
     return ((SR_RENDER_LIGHT_TREE if light_tree else 0) | (SR_RENDER_SUBSURFACE if subsurface else 0)
             | (SR_RENDER_END_DEAD_PATHS if end_dead_paths else 0))
+
+
+CHECKPOINT_VERSION = 1
+
+
+def check_checkpoint(meta, scene_sha256, bands, accum_shape):
+    """Raise ValueError unless a checkpoint's metadata belongs to this scene
+    (geometry, materials, camera and band grid, by digest) and its
+    accumulator has the [W, H, B] shape it records."""
+    if meta.get("version") != CHECKPOINT_VERSION:
+        raise ValueError(f"checkpoint version {meta.get('version')!r}, expected {CHECKPOINT_VERSION}")
+    if meta.get("scene_sha256") != scene_sha256:
+        raise ValueError("checkpoint was rendered from another scene (geometry, materials, camera or band grid)")
+    if meta.get("bands") != bands:
+        raise ValueError(f"checkpoint has {meta.get('bands')} bands, the renderer {bands}")
+    want = (meta.get("width"), meta.get("height"), meta.get("bands"))
+    if tuple(accum_shape) != want:
+        raise ValueError(f"checkpoint accumulator shape {tuple(accum_shape)} is not {want}")
+    if not isinstance(meta.get("blocks_done"), int) or meta["blocks_done"] < 0:
+        raise ValueError("checkpoint blocks_done must be a non-negative integer")
+    if meta["blocks_done"] > 0 and not isinstance(meta.get("render"), dict):
+        raise ValueError("checkpoint of a started render carries no render arguments")
 
 
 class Renderer:
@@ -65,6 +89,7 @@ class Renderer:
         self.accum = None
         self.width = self.height = 0
         self.blocks_done = 0
+        self._render_args = None  # the estimator the accumulated blocks belong to
         if width is not None:
             self.set_resolution(width, height if height is not None else width)
 
@@ -142,12 +167,14 @@ class Renderer:
         self.width, self.height = int(width), int(height)
         self.accum = torch.zeros((self.width, self.height, self.bands), dtype=torch.float32, device=self.device)
         self.blocks_done = 0
+        self._render_args = None
 
     def clear_pixel_space(self):
         """clear_pixel_space + counter reset (main.py:1661-1674, 1819-1822)."""
         if self.accum is not None:
             self.accum.zero_()
         self.blocks_done = 0
+        self._render_args = None
 
     def update_camera(self, origin, lookat, up):
         """update_camera (main.py:633-640) followed by the reset the reference
@@ -177,7 +204,55 @@ class Renderer:
                   int(n_blocks), int(max_depth), render_flags(light_tree, subsurface, end_dead_paths),
                   _lib.ptr(self.accum))
         self.blocks_done += int(n_blocks)
+        self._render_args = {"seed": int(seed), "stream": int(stream), "max_depth": int(max_depth),
+                             "light_tree": bool(light_tree), "subsurface": bool(subsurface),
+                             "end_dead_paths": bool(end_dead_paths)}
         return self.accum
+
+    # ------------------------------------------------------------ checkpoint / resume
+    # No counterpart in the reference: its accumulators live only in device
+    # fields and are cleared on every camera or mode change (main.py:1661-1674,
+    # 1812-1823).  Paths draw their uniforms from counters keyed by (seed,
+    # stream, pixel, block, band, segment), so blocks [n, n + k) rendered
+    # after a resume are the blocks an uninterrupted render would have run.
+    def scene_digest(self):
+        """sha256 of the uploaded scene (triangles, normals, materials, EEM
+        tables, camera) and of the band grid."""
+        h = hashlib.sha256()
+        f = self.flat
+        for a in (f.tris_v, f.tris_n, f.tri_mat, f.mat_f32, f.mat_i32, f.mat_eem, f.eem, f.cam):
+            a = np.ascontiguousarray(a)
+            h.update(f"{a.dtype.str}{a.shape}".encode())
+            h.update(a.tobytes())
+        h.update(np.array([self.bands, self.wl_min, self.wl_max], np.float64).tobytes())
+        return h.hexdigest()
+
+    def checkpoint(self, path):
+        """Write the running mean, blocks_done and the render arguments of
+        the accumulated blocks to ``path`` (numpy .npz, no pickled objects)."""
+        if self.accum is None:
+            raise RuntimeError("set_resolution() first")
+        meta = {"version": CHECKPOINT_VERSION, "blocks_done": int(self.blocks_done), "width": self.width,
+                "height": self.height, "bands": self.bands, "scene_sha256": self.scene_digest(),
+                "render": self._render_args}
+        with open(path, "wb") as fh:
+            np.savez(fh, accum=self.accum.detach().cpu().numpy(), meta=np.array(json.dumps(meta)))
+
+    def resume(self, path):
+        """Load a checkpoint of this scene: the resolution, running mean and
+        blocks_done are restored, and the render arguments of its blocks are
+        returned (pass them to ``render`` to continue the same estimate;
+        None if no block was rendered).  ValueError if the checkpoint
+        belongs to another scene or band grid."""
+        with np.load(path, allow_pickle=False) as z:
+            meta = json.loads(str(z["meta"]))
+            acc = np.ascontiguousarray(z["accum"], np.float32)
+        check_checkpoint(meta, self.scene_digest(), self.bands, acc.shape)
+        self.set_resolution(meta["width"], meta["height"])
+        self.accum.copy_(torch.from_numpy(acc))
+        self.blocks_done = meta["blocks_done"]
+        self._render_args = meta["render"]
+        return None if self._render_args is None else dict(self._render_args)
 
     def color_rays(self, colour_system, exposure=1.0, gamma=2.0, out_width=None, out_height=None):
         """color_rays (main.py:1609-1629) -> [out_w, out_h, 3] tensor."""

@@ -2,8 +2,9 @@
 change orders the handle's work (sr_set_stream), a batch launched again
 continues from where its scenes are (sr_batch_* with Batch.run) and sees a
 changed camera, sr_update_materials refuses a change of surface_emission
-(the light lists were built from it), and RenderFunction's backward replays
-the paths of its own forward even after another forward ran."""
+(the light lists were built from it), RenderFunction's backward replays
+the paths of its own forward even after another forward ran, and a render
+resumed from a checkpoint in a fresh renderer continues the same estimate."""
 import ctypes
 
 import numpy as np
@@ -139,3 +140,22 @@ def test_autograd_backward_replays_its_own_forward(gpu):
     assert torch.allclose(b1.grad, alone, rtol=1e-4, atol=1e-6)
     (out2 * d).sum().backward()
     assert not torch.allclose(b2.grad, alone, rtol=1e-3, atol=1e-6)
+
+
+def test_resume_continues_the_same_estimate(gpu, tmp_path):
+    c, f = _box()
+    a = _new(c, f, gpu)
+    a.render(n_blocks=3, max_depth=c.depth, seed=11, stream=1)
+    a.checkpoint(tmp_path / "ck.npz")
+    a.render(n_blocks=2, max_depth=c.depth, seed=11, stream=1)
+    b = Renderer(f, c.bands, c.wl_min, c.wl_max, device=gpu)  # a fresh handle, no resolution yet
+    args = b.resume(tmp_path / "ck.npz")
+    assert args == {"seed": 11, "stream": 1, "max_depth": c.depth, "light_tree": False, "subsurface": False,
+                    "end_dead_paths": False}
+    assert (b.width, b.height, b.blocks_done) == (c.width, c.height, 3)
+    b.render(n_blocks=2, **args)
+    assert b.blocks_done == 5
+    # blocks 3 and 4 are the same paths in both; only the order of the float
+    # atomics into the block sums can differ
+    np.testing.assert_allclose(b.accum.cpu().numpy(), a.accum.cpu().numpy(), rtol=1e-5, atol=1e-7)
+    assert float(a.accum.abs().sum()) > 0.0
