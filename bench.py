@@ -470,6 +470,8 @@ def main():
     ap.add_argument("--no-dead", action="store_true",
                     help="leave out the end_dead_paths key (the same blocks with dead paths ended)")
     ap.add_argument("--no-inverse", action="store_true", help="leave the inverse_render key out")
+    ap.add_argument("--no-full", action="store_true",
+                    help="leave the full_render key (the configuration's whole render in one launch) out")
     ap.add_argument("--spp-scale", type=int, default=8, help="batch64: divide each scene's spp by this")
     ap.add_argument("--build", choices=("host", "device"), default="host",
                     help="batch64: where the scenes are compiled")
@@ -605,6 +607,20 @@ def main():
                                  "Mseg_per_s": round(st_d["segments"] / wall_d / 1e6, 2),
                                  "block_time_ratio": round(wall_d / wall, 4),
                                  "nonfinite": st_d["nonfinite"]}
+    if not args.no_full and cfg.spp > args.steps:
+        # the configuration's whole render -- its spp spectral blocks (64 for
+        # the 1M-triangle heightfield) -- as the one launch it runs as
+        r.clear_pixel_space()
+        torch.cuda.synchronize()
+        t0 = time.time()
+        r.render(n_blocks=cfg.spp, max_depth=cfg.depth, seed=cfg.seed)
+        torch.cuda.synchronize()
+        wall_f = time.time() - t0
+        st_f = r.stats()
+        out["full_render"] = {"blocks": cfg.spp, "seconds": round(wall_f, 3),
+                              "kernel_seconds": round(r.last_kernel_ms() / 1e3, 3),
+                              "segments": st_f["segments"], "Mseg_per_s": round(st_f["segments"] / wall_f / 1e6, 2),
+                              "ms_per_block": round(wall_f * 1e3 / cfg.spp, 3), "nonfinite": st_f["nonfinite"]}
     del r
     if not args.no_adjoint:
         out["adjoint_forward_ratio"] = adjoint_ratio()
