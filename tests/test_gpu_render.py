@@ -204,6 +204,35 @@ def test_resolve_matches_color_rays_closed_form(box_small, gpu, W, H, OW, OH):
         assert np.array_equal(out, base[ri32][:, rj32], equal_nan=True)
 
 
+@pytest.mark.parametrize("OW,OH", [(33, 9), (66, 18)])
+def test_resolve_two_bands(gpu, OW, OH):
+    """The fewest bands a handle takes (2), through the LDS-staged resolve
+    tile over a partial last tile, at the render grid and upsampled."""
+    import torch
+
+    from spectral.data import srgb_colour_system
+
+    cfg = configs.box_icosphere(bands=2)
+    flat = flatten_scene(cfg.scene, cfg.bands, cfg.wl_min, cfg.wl_max)
+    W, H = 33, 9
+    r = _renderer(flat, cfg, W, H, gpu)
+    acc = np.random.default_rng(1).uniform(0, 2, (W, H, 2)).astype(np.float32)
+    r.accum.copy_(torch.from_numpy(acc))
+    cs = srgb_colour_system(cfg.wl_min, cfg.wl_max, 2)
+    out = r.color_rays(cs, exposure=1.0, gamma=2.0, out_width=OW, out_height=OH).cpu().numpy()
+    step = cfg.wl_max - cfg.wl_min
+    ri = (np.arange(OW) / OW * W).astype(int)
+    rj = (np.arange(OH) / OH * H).astype(int)
+    xyz = acc[ri][:, rj].astype(np.float64) @ cs.cmf * (step / (np.sum(cs.cmf[:, 1]) * step))
+    rgb = xyz @ cs.T.T
+    with np.errstate(invalid="ignore"):  # a negative channel is NaN, as in the kernel
+        ldr = np.power(rgb / (rgb + 1.0), 0.5)
+    fin = np.isfinite(ldr)
+    assert fin.any()
+    assert np.allclose(out[fin], ldr[fin], rtol=1e-4, atol=1e-5)
+    assert np.array_equal(np.isnan(out), ~fin)
+
+
 def test_upsampled_resolve_after_resizing(box_small, gpu):
     """One handle, render grids of growing size: the display-size scratch
     image grows with them and every upsampled resolve is the gathered
