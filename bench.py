@@ -95,8 +95,11 @@ def cpu_baseline(flat, cfg, target_s=15.0):
 def adjoint_ratio(reps=2):
     """adjoint : forward device-time ratio on the inverse-render workload
     (volume/fluorescence scene, 256x256, 64 spp, depth 12, 126 parameters):
-    path kernel time of sr_render_adjoint over that of sr_render for the
-    same blocks and seed."""
+    path kernel time of the adjoint over that of the differentiable forward
+    for the same blocks and seed.  The differentiable forward records each
+    path's total (sr_render_record) and the adjoint takes it in place of its
+    first pass (sr_render_adjoint_record), as RenderFunction runs them; the
+    two-pass adjoint and the plain forward are reported beside them."""
     import torch
 
     from spectral import configs
@@ -108,23 +111,33 @@ def adjoint_ratio(reps=2):
     ds = DifferentiableScene(flat, inverse_volume_spec(), cfg.width, cfg.height, cfg.spp, cfg.depth, cfg.seed,
                              device="cuda:0")
     d_acc = torch.rand((cfg.width, cfg.height, cfg.bands), device="cuda:0")
-    fwd, adj, full = [], [], []
+    tot = ds.new_path_totals()
+    fwd, adj, plain, adj2, full = [], [], [], [], []
     for _ in range(reps + 1):
-        ds.render()  # the differentiable forward: dead paths ended, as the adjoint replays them
+        ds.render(path_totals=tot)  # the differentiable forward: dead paths ended, as the adjoint replays them
         fwd.append(ds.r.last_kernel_ms())
         segs = ds.r.stats()["segments"]
-        ds.adjoint(d_acc)
+        ds.adjoint(d_acc, path_totals=tot)
         adj.append(ds.r.last_kernel_ms())
+        ds.render()  # the same forward without the record
+        plain.append(ds.r.last_kernel_ms())
+        ds.adjoint(d_acc)  # every path replayed twice
+        adj2.append(ds.r.last_kernel_ms())
         ds.r.clear_pixel_space()  # the same image with every path run to the depth limit
         ds.r.render(n_blocks=ds.spp, max_depth=ds.depth, seed=ds.seed)
         full.append(ds.r.last_kernel_ms())
-    fwd, adj, full = min(fwd[1:]), min(adj[1:]), min(full[1:])
+    fwd, adj, plain, adj2, full = (min(x[1:]) for x in (fwd, adj, plain, adj2, full))
+    fluoresced = torch.isnan(tot).float().mean().item()
     return {"value": round(adj / fwd, 3), "forward_ms": round(fwd, 2), "adjoint_ms": round(adj, 2),
+            "two_pass_adjoint_ms": round(adj2, 2), "two_pass_ratio": round(adj2 / plain, 3),
+            "forward_without_record_ms": round(plain, 2),
             "adjoint_over_full_forward": round(adj / full, 3), "full_forward_ms": round(full, 2),
+            "paths_replayed_twice": round(fluoresced, 4),
             "workload": f"{cfg.name}: {flat.n_tris} tris, {cfg.bands} bands, {cfg.width}x{cfg.height}, "
                         f"{cfg.spp} spp, depth {cfg.depth}, {ds.n_params} params ({segs} segments per forward; "
                         f"forward and adjoint end paths whose throughput is 0, full_forward runs every path "
-                        f"to the depth limit)"}
+                        f"to the depth limit; the forward records each path's total and the adjoint replays "
+                        f"once every path whose light stayed in its band)"}
 
 
 def bench_inverse(iterations=3, width=256, height=256, spp=64, log=print):

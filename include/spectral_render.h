@@ -207,6 +207,25 @@ int sr_render_adjoint(sr_handle* h, int width, int height, uint64_t seed, uint32
                       int rough_idx, int g_mat, int g_idx, int sig_mat, int sig_off, int n_params, int flags,
                       float* dev_grad);
 
+/* sr_render that also records each path's total for the adjoint (no
+ * reference counterpart).  dev_path_totals [n_blocks][W][H][B] f32 (device)
+ * receives, for the path of block first_block + k, pixel (i, j) and band b,
+ * the sum of its contributions -- or NaN when fluorescence sent some of
+ * them to another band.  flags: 0 or SR_RENDER_END_DEAD_PATHS (the adjoint
+ * replays the area-CDF light pick without the subsurface walk). */
+int sr_render_record(sr_handle* h, int width, int height, uint64_t seed, uint32_t stream, int first_block,
+                     int n_blocks, int max_depth, int flags, float* dev_accum, float* dev_path_totals);
+
+/* sr_render_adjoint given the totals an sr_render_record call with the same
+ * scene, parameters, seed, stream and blocks wrote: a path with a finite
+ * total is replayed once instead of twice (its first pass only sums
+ * dL/dacc x contribution over the path).  Same arguments and gradient as
+ * sr_render_adjoint otherwise. */
+int sr_render_adjoint_record(sr_handle* h, int width, int height, uint64_t seed, uint32_t stream, int first_block,
+                             int n_blocks, int max_depth, const float* dev_d_accum, const int* alb_off,
+                             int rough_mat, int rough_idx, int g_mat, int g_idx, int sig_mat, int sig_off,
+                             int n_params, int flags, const float* dev_path_totals, float* dev_grad);
+
 /* color_rays (main.py:1609-1629): dev_accum [W][H][B] -> dev_rgb
  * [out_w][out_h][3] with nearest upsampling.  cmf [B][3] (host),
  * xyz2rgb [3][3] row-major (host), norm = sum(cmf_y) * step. */

@@ -751,10 +751,11 @@ struct LaunchBufs {
 
 static int prepare_path_args(sr_handle* h, PathArgs& a, const LaunchBufs& lb, int width, int height, uint64_t seed,
                              uint32_t stream, int first_block, int n_blocks, int max_depth, int flags = 0,
-                             bool adjoint = false) {
+                             bool adjoint = false, float* rec = nullptr) {
   hipStream_t st = h->st;
   fill_job(h, h->job_host, width, height, seed, stream, first_block, n_blocks, max_depth);
   h->job_host.sum = lb.sum;
+  h->job_host.rec = rec;
   h->job_host.nee_tree = ((flags & SR_RENDER_LIGHT_TREE) && h->n_lnodes > 0) ? 1 : 0;
   h->job_host.sss = (flags & SR_RENDER_SUBSURFACE) ? 1 : 0;
   h->job_host.end_dead = (flags & SR_RENDER_END_DEAD_PATHS) ? 1 : 0;
@@ -788,9 +789,11 @@ static LaunchBufs handle_bufs(sr_handle* h) {
 // The path launch and the fold of its blocks into dev_accum (sr_render and
 // sr_render_noalloc; the arguments are checked by the caller).
 static int render_with(sr_handle* h, const LaunchBufs& lb, int width, int height, uint64_t seed, uint32_t stream,
-                       int first_block, int n_blocks, int max_depth, int flags, float* dev_accum) {
+                       int first_block, int n_blocks, int max_depth, int flags, float* dev_accum,
+                       float* rec = nullptr) {
   PathArgs a;
-  int rc = prepare_path_args(h, a, lb, width, height, seed, stream, first_block, n_blocks, max_depth, flags);
+  int rc = prepare_path_args(h, a, lb, width, height, seed, stream, first_block, n_blocks, max_depth, flags, false,
+                             rec);
   if (rc) return rc;
   hipStream_t st = h->st;
   const bool sss = (flags & SR_RENDER_SUBSURFACE) != 0;
@@ -822,6 +825,23 @@ int sr_render(sr_handle* h, int width, int height, uint64_t seed, uint32_t strea
   if ((rc = ensure_sum(h, width, height, h->st))) return rc;
   return render_with(h, handle_bufs(h), width, height, seed, stream, first_block, n_blocks, max_depth, flags,
                      dev_accum);
+}
+
+int sr_render_record(sr_handle* h, int width, int height, uint64_t seed, uint32_t stream, int first_block,
+                     int n_blocks, int max_depth, int flags, float* dev_accum, float* dev_path_totals) {
+  g_err.clear();
+  if (!h) return fail(SR_EINVAL, "null handle");
+  if (flags & ~SR_RENDER_END_DEAD_PATHS)
+    return fail(SR_EINVAL, "sr_render_record takes only SR_RENDER_END_DEAD_PATHS (flags 0x%x): the adjoint "
+                           "replays the area-CDF light pick without the subsurface walk", flags);
+  int rc = check_render_args(h, width, height, first_block, n_blocks, max_depth, stream);
+  if (rc) return rc;
+  if (!dev_accum || !dev_path_totals) return fail(SR_EINVAL, "dev_accum or dev_path_totals is NULL");
+  HIPCHK(hipSetDevice(h->device));
+  if (n_blocks == 0) return 0;
+  if ((rc = ensure_sum(h, width, height, h->st))) return rc;
+  return render_with(h, handle_bufs(h), width, height, seed, stream, first_block, n_blocks, max_depth, flags,
+                     dev_accum, dev_path_totals);
 }
 
 // ---------------------------------------------------------------- allocation-free render
@@ -881,11 +901,10 @@ int sr_render_noalloc(sr_handle* h, int width, int height, uint64_t seed, uint32
   return render_with(h, lb, width, height, seed, stream, first_block, n_blocks, max_depth, flags, dev_accum);
 }
 
-int sr_render_adjoint(sr_handle* h, int width, int height, uint64_t seed, uint32_t stream, int first_block,
-                      int n_blocks, int max_depth, const float* dev_d_accum, const int* alb_off, int rough_mat,
-                      int rough_idx, int g_mat, int g_idx, int sig_mat, int sig_off, int n_params, int flags,
-                      float* dev_grad) {
-  g_err.clear();
+static int adjoint_with(sr_handle* h, int width, int height, uint64_t seed, uint32_t stream, int first_block,
+                        int n_blocks, int max_depth, const float* dev_d_accum, const int* alb_off, int rough_mat,
+                        int rough_idx, int g_mat, int g_idx, int sig_mat, int sig_off, int n_params, int flags,
+                        const float* rec, float* dev_grad) {
   int rc = check_render_args(h, width, height, first_block, n_blocks, max_depth, stream);
   if (rc) return rc;
   if (!dev_d_accum || !dev_grad || !alb_off) return fail(SR_EINVAL, "null array");
@@ -907,7 +926,7 @@ int sr_render_adjoint(sr_handle* h, int width, int height, uint64_t seed, uint32
   // derivative, and R (the radiance still to come) and the contributions are
   // exactly 0 there, so the gradient is the same whether or not they run.
   if ((rc = prepare_path_args(h, a, handle_bufs(h), width, height, seed, stream, first_block, n_blocks, max_depth,
-                               SR_RENDER_END_DEAD_PATHS, true)))
+                               SR_RENDER_END_DEAD_PATHS, true, const_cast<float*>(rec))))
     return rc;
   if ((rc = h->alb_off.upload(alb_off, (size_t)M, h->st))) return rc;
   a.adj.d_accum = dev_d_accum;
@@ -933,6 +952,27 @@ int sr_render_adjoint(sr_handle* h, int width, int height, uint64_t seed, uint32
   h->timed = true;
   HIPCHK(hipStreamSynchronize(st));  // alb_off staging buffer is reused by the next call
   return 0;
+}
+
+int sr_render_adjoint(sr_handle* h, int width, int height, uint64_t seed, uint32_t stream, int first_block,
+                      int n_blocks, int max_depth, const float* dev_d_accum, const int* alb_off, int rough_mat,
+                      int rough_idx, int g_mat, int g_idx, int sig_mat, int sig_off, int n_params, int flags,
+                      float* dev_grad) {
+  g_err.clear();
+  return adjoint_with(h, width, height, seed, stream, first_block, n_blocks, max_depth, dev_d_accum, alb_off,
+                      rough_mat, rough_idx, g_mat, g_idx, sig_mat, sig_off, n_params, flags, nullptr, dev_grad);
+}
+
+int sr_render_adjoint_record(sr_handle* h, int width, int height, uint64_t seed, uint32_t stream, int first_block,
+                             int n_blocks, int max_depth, const float* dev_d_accum, const int* alb_off,
+                             int rough_mat, int rough_idx, int g_mat, int g_idx, int sig_mat, int sig_off,
+                             int n_params, int flags, const float* dev_path_totals, float* dev_grad) {
+  g_err.clear();
+  if (!h) return fail(SR_EINVAL, "null handle");
+  if (!dev_path_totals) return fail(SR_EINVAL, "dev_path_totals is NULL");
+  return adjoint_with(h, width, height, seed, stream, first_block, n_blocks, max_depth, dev_d_accum, alb_off,
+                      rough_mat, rough_idx, g_mat, g_idx, sig_mat, sig_off, n_params, flags, dev_path_totals,
+                      dev_grad);
 }
 
 int sr_render_normals(sr_handle* h, int width, int height, int out_w, int out_h, float* dev_rgb) {

@@ -100,6 +100,11 @@ struct Job {
   int sss;           // 1: subsurface random walk (SR_RENDER_SUBSURFACE; the walk kernel only)
   int end_dead;      // 1: a path ends once its throughput is exactly 0 (SR_RENDER_END_DEAD_PATHS)
   uint32_t per_block;  // items per spectral block: W * H * items per pixel
+  // Path totals [n_blocks][W][H][B] (null: none).  The forward writes each
+  // path's summed contributions there, or NaN when some of them went to
+  // another band (fluorescence); the adjoint takes a finite total as its
+  // first pass and replays only the other paths twice.
+  float* rec;
   // queue x: items [q_off[x], q_off[x] + q_len[x]) of every spectral block
   // (whole 64-item tile groups when tiled), q_len[x] * n_blocks in all
   uint32_t q_off[SR_QUEUES], q_len[SR_QUEUES];

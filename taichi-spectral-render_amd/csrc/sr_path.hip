@@ -144,13 +144,24 @@ __device__ __forceinline__ void emit_grad(const Ctx<ADJ>& c, int idx, float v) {
   if (idx >= 0 && v != 0.0f && isfinite(v)) atomicAdd(c.lds_grad + idx, v);
 }
 
+// L.acc_band: the band L.acc is summed for, with SR_ACC_SWITCHED set once
+// a contribution of the path went to another band than the path's own.
+#define SR_ACC_SWITCHED 0x100
+#define SR_ACC_BAND_MASK 0xFF
+
 template <bool ADJ>
 __device__ __forceinline__ void flush_acc(Lane& L, const Ctx<ADJ>& c) {
   if (!ADJ && L.acc != 0.0f) {
-    size_t idx = ((size_t)L.pi * c.a.H + L.pj) * c.a.s.B + L.acc_band;
+    size_t idx = ((size_t)L.pi * c.a.H + L.pj) * c.a.s.B + (L.acc_band & SR_ACC_BAND_MASK);
     atomicAdd(c.a.sum + idx, L.acc);
   }
   L.acc = 0.0f;
+}
+
+// Index of the lane's path in Job::rec: [block - first_block][i][j][band].
+template <bool ADJ>
+__device__ __forceinline__ size_t rec_index(const Lane& L, const Ctx<ADJ>& c) {
+  return (((size_t)(L.blk - c.a.first_block) * c.a.W + L.pi) * c.a.H + L.pj) * c.a.s.B + L.band;
 }
 
 // block_accumulator[i, j, active] += contribution (main.py:1589-1590);
@@ -168,15 +179,18 @@ __device__ __forceinline__ void add_contrib(Lane& L, const Ctx<ADJ>& c, float v)
     else L.Eb += w;
     return;
   }
-  if (L.active != L.acc_band) {
+  if (L.active != (L.acc_band & SR_ACC_BAND_MASK)) {
     flush_acc<ADJ>(L, c);
-    L.acc_band = L.active;
+    L.acc_band = L.active | SR_ACC_SWITCHED;
   }
   L.acc += v;
 }
 
 template <bool ADJ>
 __device__ __forceinline__ void end_path(Lane& L, const Ctx<ADJ>& c) {
+  // the path's total for the adjoint: L.acc holds every contribution of
+  // the path unless one went to another band
+  if (!ADJ && c.a.rec) c.a.rec[rec_index(L, c)] = (L.acc_band & SR_ACC_SWITCHED) ? NAN : L.acc;
   flush_acc<ADJ>(L, c);
   if (ADJ && L.pass == 0) {
     L.pass = 1;  // replay the same path for the gradient pass
@@ -212,7 +226,19 @@ __device__ __forceinline__ void start_path(Lane& L, const Ctx<ADJ>& c) {
   L.wstep = 0;
   L.wmat = 0;
   if (ADJ) {
-    if (L.pass == 0) L.Lw = 0.0f;
+    if (L.pass == 0) {
+      L.Lw = 0.0f;
+      // A total recorded by the forward replaces the first pass: pass 0
+      // would sum dL/dacc[band] x c_k over the same contributions c_k, all
+      // in the path's own band.
+      if (a.rec) {
+        const float tot = a.rec[rec_index(L, c)];
+        if (isfinite(tot)) {
+          L.Lw = dacc(c, L, L.band) * tot;
+          L.pass = 1;
+        }
+      }
+    }
     L.Eb = 0.0f;
     L.sp_dr = 0.0f;
   }

@@ -71,6 +71,9 @@ class DifferentiableScene:
         for m in spec.albedo_mats:
             self.alb_off[m] = self.offsets[("albedo", m)]
         self.current_params = None  # the parameters last uploaded by set_params
+        # RenderFunction records the forward's path totals (4 bytes per path)
+        # up to this size; larger renders replay every path twice
+        self.record_limit_bytes = 4 << 30
 
     def initial_params(self):
         """Parameter vector read back from the current material tables."""
@@ -105,16 +108,29 @@ class DifferentiableScene:
         self.r.update_materials(f.astype(np.float32))
         self.current_params = th.copy()
 
-    def render(self, stream=0):
+    def new_path_totals(self):
+        """A buffer for the path totals of one render (one float per path),
+        or None when it would exceed ``record_limit_bytes``."""
+        n = self.spp * self.r.width * self.r.height * self.B
+        if n * 4 > self.record_limit_bytes:
+            return None
+        return torch.empty((n,), dtype=torch.float32, device=self.r.accum.device)
+
+    def render(self, stream=0, path_totals=None):
         """Fresh render of `spp` blocks; returns the spectral mean [W, H, B].
         Paths end once their throughput is exactly 0 (the same image; the
-        adjoint replays them the same way)."""
+        adjoint replays them the same way).  ``path_totals`` (from
+        new_path_totals) receives each path's total for ``adjoint``."""
         self.r.clear_pixel_space()
-        self.r.render(n_blocks=self.spp, max_depth=self.depth, seed=self.seed, stream=stream, end_dead_paths=True)
+        self.r.render(n_blocks=self.spp, max_depth=self.depth, seed=self.seed, stream=stream, end_dead_paths=True,
+                      path_totals=path_totals)
         return self.r.accum
 
-    def adjoint(self, d_accum, stream=0, flags=0):
-        """Gradient of sum(d_accum * render()) with respect to the parameters."""
+    def adjoint(self, d_accum, stream=0, flags=0, path_totals=None):
+        """Gradient of sum(d_accum * render()) with respect to the parameters.
+        With the ``path_totals`` of that render, each path whose
+        contributions all stayed in its band is replayed once instead of
+        twice (the same gradient up to float rounding)."""
         d = d_accum.contiguous().to(torch.float32)
         want = (self.r.width, self.r.height, self.B)
         if tuple(d.shape) != want or d.device.type != "cuda":
@@ -122,10 +138,17 @@ class DifferentiableScene:
         grad = torch.zeros((self.n_params,), dtype=torch.float32, device=d.device)
         sp = self.spec
         self.r._bind_stream()
-        _lib.call("sr_render_adjoint", self.r._h, self.r.width, self.r.height, ctypes.c_uint64(self.seed),
-                  ctypes.c_uint32(stream), 0, self.spp, self.depth, _lib.ptr(d), _lib.ptr(self.alb_off),
-                  sp.rough_mat, self.offsets.get("roughness", -1), sp.g_mat, self.offsets.get("g", -1),
-                  sp.sigma_mat, self.offsets.get("sigma_a", -1), self.n_params, int(flags), _lib.ptr(grad))
+        args = [self.r._h, self.r.width, self.r.height, ctypes.c_uint64(self.seed), ctypes.c_uint32(stream), 0,
+                self.spp, self.depth, _lib.ptr(d), _lib.ptr(self.alb_off), sp.rough_mat,
+                self.offsets.get("roughness", -1), sp.g_mat, self.offsets.get("g", -1), sp.sigma_mat,
+                self.offsets.get("sigma_a", -1), self.n_params, int(flags)]
+        if path_totals is None:
+            _lib.call("sr_render_adjoint", *args, _lib.ptr(grad))
+        else:
+            n = self.spp * self.r.width * self.r.height * self.B
+            if path_totals.numel() != n or path_totals.dtype != torch.float32 or path_totals.device != d.device:
+                raise ValueError(f"path_totals must be a float32 tensor of {n} elements on {d.device}")
+            _lib.call("sr_render_adjoint_record", *args, _lib.ptr(path_totals), _lib.ptr(grad))
         return grad
 
 
@@ -138,8 +161,10 @@ class RenderFunction(torch.autograd.Function):
     def forward(ctx, theta, scene: DifferentiableScene, stream=0):
         th = theta.detach().cpu().numpy().copy()
         scene.set_params(th)
-        out = scene.render(stream=stream).clone()
+        totals = scene.new_path_totals()
+        out = scene.render(stream=stream, path_totals=totals).clone()
         ctx.scene = scene
+        ctx.path_totals = totals  # this forward's paths: the adjoint replays them once
         ctx.theta = th  # the adjoint replays the paths of THESE parameters
         ctx.stream = int(stream)
         ctx.seed = scene.seed
@@ -155,7 +180,7 @@ class RenderFunction(torch.autograd.Function):
         seed = scene.seed
         scene.seed = ctx.seed
         try:
-            g = scene.adjoint(grad_out, stream=ctx.stream)
+            g = scene.adjoint(grad_out, stream=ctx.stream, path_totals=ctx.path_totals)
         finally:
             scene.seed = seed
         return g.to(device=ctx.theta_device, dtype=ctx.theta_dtype), None, None

@@ -185,7 +185,7 @@ class Renderer:
         self.clear_pixel_space()
 
     def render(self, n_blocks=1, max_depth=7, seed=0, stream=0, light_tree=False, subsurface=False,
-               end_dead_paths=False):
+               end_dead_paths=False, path_totals=None):
         """Run blocks [blocks_done, blocks_done + n_blocks) and fold them into
         the running spectral mean ``accum`` [W, H, B] (cast_rays x B +
         average_and_reset_block per block, main.py:1826-1836).
@@ -196,13 +196,27 @@ class Renderer:
         scattering coefficient scatter by a random walk inside their (closed)
         mesh instead of the single-scatter Oren-Nayar lobe.  ``end_dead_paths``:
         a path ends once its throughput is exactly 0 instead of running to
-        ``max_depth`` (the same image; fewer segments executed and counted)."""
+        ``max_depth`` (the same image; fewer segments executed and counted).
+        ``path_totals``: a float32 tensor on the render's device with
+        n_blocks x W x H x B elements that receives each path's summed
+        contributions (NaN where fluorescence moved some of them to another
+        band) for the adjoint; not with ``light_tree`` or ``subsurface``."""
         if self.accum is None:
             raise RuntimeError("set_resolution() first")
         self._bind_stream()
-        _lib.call("sr_render", self._h, self.width, self.height, int(seed), int(stream), self.blocks_done,
-                  int(n_blocks), int(max_depth), render_flags(light_tree, subsurface, end_dead_paths),
-                  _lib.ptr(self.accum))
+        flags = render_flags(light_tree, subsurface, end_dead_paths)
+        if path_totals is None:
+            _lib.call("sr_render", self._h, self.width, self.height, int(seed), int(stream), self.blocks_done,
+                      int(n_blocks), int(max_depth), flags, _lib.ptr(self.accum))
+        else:
+            n = int(n_blocks) * self.width * self.height * self.bands
+            if (path_totals.dtype != torch.float32 or path_totals.device != self.accum.device
+                    or not path_totals.is_contiguous() or path_totals.numel() != n):
+                raise ValueError(f"path_totals must be a contiguous float32 tensor of {n} elements on "
+                                 f"{self.accum.device}")
+            _lib.call("sr_render_record", self._h, self.width, self.height, int(seed), int(stream),
+                      self.blocks_done, int(n_blocks), int(max_depth), flags, _lib.ptr(self.accum),
+                      _lib.ptr(path_totals))
         self.blocks_done += int(n_blocks)
         self._render_args = {"seed": int(seed), "stream": int(stream), "max_depth": int(max_depth),
                              "light_tree": bool(light_tree), "subsurface": bool(subsurface),

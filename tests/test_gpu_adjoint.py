@@ -32,8 +32,14 @@ NOISE_BOUND = 0.025  # 3 standard errors, relative to the derivative
 BIAS_BOUND = 0.05
 
 
-def _scene(gpu, W, spp):
+def _scene(gpu, W, spp, fluorescent_medium=False):
     cfg = configs.inverse_volume(width=W, height=W, spp=spp, depth=12, seed=3)
+    if fluorescent_medium:
+        # The slab's fluorescent light never leaves the slab under the
+        # reference's rules (its inner surface sees n_vol = n_surf, so the
+        # rough BTDF gives 0 or 0/0 there): the index-matched medium sphere
+        # gets the EEM as well, so that some paths move light to other bands.
+        cfg.scene["materials"][5]["eem_id"] = 1
     flat = flatten_scene(cfg.scene, cfg.bands, cfg.wl_min, cfg.wl_max)
     ds = DifferentiableScene(flat, inverse_volume_spec(), cfg.width, cfg.height, cfg.spp, cfg.depth, cfg.seed,
                              device=gpu)
@@ -124,3 +130,67 @@ def test_autograd_function(setup):
     (out * d_acc).sum().backward()
     assert th.grad is not None and th.grad.shape == (126,)
     assert np.allclose(th.grad.numpy(), g_adj[0], rtol=1e-4, atol=1e-6)
+
+
+def test_path_totals_are_the_block_images(gpu):
+    """sr_render_record writes one total per path at [block][i][j][band]: on
+    the box scene (no fluorescence, so every total is finite and is that
+    path's whole contribution) each block's totals are the image of that
+    block rendered alone, and recording leaves the image unchanged."""
+    from spectral.render import Renderer
+
+    W = H = 16
+    nb = 3
+    cfg = configs.box_icosphere(width=W, height=H, spp=nb, depth=8)
+    flat = flatten_scene(cfg.scene, cfg.bands, cfg.wl_min, cfg.wl_max)
+    B = cfg.bands
+    r = Renderer(flat, B, cfg.wl_min, cfg.wl_max, width=W, height=H, device=gpu)
+    tot = torch.full((nb * W * H * B,), -1.0, device=gpu)
+    r.render(n_blocks=nb, max_depth=8, seed=5, path_totals=tot)
+    img_rec = r.accum.clone()
+    r.clear_pixel_space()
+    r.render(n_blocks=nb, max_depth=8, seed=5)
+    torch.testing.assert_close(img_rec, r.accum, rtol=1e-6, atol=1e-7)
+    tot = tot.view(nb, W, H, B)
+    assert torch.isfinite(tot).all() and (tot >= 0).all()
+    assert (tot > 0).float().mean().item() > 0.3
+    for b in range(nb):
+        r.clear_pixel_space()
+        r.blocks_done = b  # block b alone: accum = block image / (b + 1)
+        r.render(n_blocks=1, max_depth=8, seed=5)
+        torch.testing.assert_close(tot[b], r.accum * (b + 1), rtol=1e-5, atol=1e-7)
+
+
+def test_recorded_adjoint_equals_two_pass(gpu):
+    """The adjoint given the forward's path totals replays each path whose
+    contributions stayed in its band once; paths that fluoresced (NaN
+    totals) are replayed twice as before.  The gradient is the two-pass one
+    up to float rounding (the first pass summed dL/dacc x c_k term by term,
+    the record gives dL/dacc x sum c_k)."""
+    ds, d_acc = _scene(gpu, 24, 64, fluorescent_medium=True)
+    ds.set_params(ds.initial_params())
+    tot = ds.new_path_totals()
+    ds.render(path_totals=tot)
+    nan_frac = torch.isnan(tot).float().mean().item()
+    assert 0.0 < nan_frac < 0.5, nan_frac  # the medium's fluorescence moves some paths' light
+    g2 = ds.adjoint(d_acc).double()
+    t2 = ds.r.last_kernel_ms()
+    g1 = ds.adjoint(d_acc, path_totals=tot).double()
+    t1 = ds.r.last_kernel_ms()
+    rel = ((g1 - g2).norm() / g2.norm()).item()
+    print(f"recorded vs two-pass adjoint: relative difference {rel:.2e}, {nan_frac:.3f} of the paths fluoresced, "
+          f"{t1:.1f} vs {t2:.1f} ms")
+    assert rel < 1e-5
+    torch.testing.assert_close(g1, g2, rtol=1e-4, atol=1e-6 * g2.abs().max().item())
+
+
+def test_record_refuses_light_tree_and_subsurface(gpu):
+    from spectral._lib import SpectralRenderError
+
+    ds, _ = _scene(gpu, 8, 2)
+    tot = ds.new_path_totals()
+    for kw in ({"light_tree": True}, {"subsurface": True}):
+        with pytest.raises(SpectralRenderError, match="SR_RENDER_END_DEAD_PATHS"):
+            ds.r.render(n_blocks=2, max_depth=4, path_totals=tot, **kw)
+    with pytest.raises(ValueError):
+        ds.r.render(n_blocks=1, max_depth=4, path_totals=tot)  # sized for 2 blocks
