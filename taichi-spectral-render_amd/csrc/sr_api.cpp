@@ -775,6 +775,7 @@ static int prepare_path_args(sr_handle* h, PathArgs& a, const LaunchBufs& lb, in
   a.overflow = lb.overflow;
   a.chunk = 64;
   a.tab_bytes = h->job_host.tab.bytes;
+  a.record = rec != nullptr ? 1u : 0u;
   a.adj.rough_mat = a.adj.g_mat = a.adj.sig_mat = -2;
   a.adj.rough_idx = a.adj.g_idx = a.adj.sig_off = -1;
   h->last_work = lb.work;

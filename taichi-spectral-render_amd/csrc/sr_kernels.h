@@ -121,6 +121,7 @@ struct PathArgs {
   uint32_t* overflow;             // traversal stack overflow flag
   uint32_t chunk;                 // items claimed per queue atomic (per wave)
   uint32_t tab_bytes;             // LDS region for the jobs' tables (every job's tab.bytes fits); 0: global
+  uint32_t record;                // 1: the kernel built to write Job::rec (a single recording job)
   AdjMap adj;                     // adjoint only (single job)
 };
 

@@ -125,6 +125,7 @@ struct Ctx {
   const AdjMap& adj;
   float* lds_grad;
   Tabs t;
+  float* rec;  // the job's path totals, or null -- a constant null in kernels built without them
 };
 
 // The lane's uniform block (segment `seg`, slot `slot`); the key is the job's
@@ -190,7 +191,7 @@ template <bool ADJ>
 __device__ __forceinline__ void end_path(Lane& L, const Ctx<ADJ>& c) {
   // the path's total for the adjoint: L.acc holds every contribution of
   // the path unless one went to another band
-  if (!ADJ && c.a.rec) c.a.rec[rec_index(L, c)] = (L.acc_band & SR_ACC_SWITCHED) ? NAN : L.acc;
+  if (!ADJ && c.rec) c.rec[rec_index(L, c)] = (L.acc_band & SR_ACC_SWITCHED) ? NAN : L.acc;
   flush_acc<ADJ>(L, c);
   if (ADJ && L.pass == 0) {
     L.pass = 1;  // replay the same path for the gradient pass
@@ -231,8 +232,8 @@ __device__ __forceinline__ void start_path(Lane& L, const Ctx<ADJ>& c) {
       // A total recorded by the forward replaces the first pass: pass 0
       // would sum dL/dacc[band] x c_k over the same contributions c_k, all
       // in the path's own band.
-      if (a.rec) {
-        const float tot = a.rec[rec_index(L, c)];
+      if (c.rec) {
+        const float tot = c.rec[rec_index(L, c)];
         if (isfinite(tot)) {
           L.Lw = dacc(c, L, L.band) * tot;
           L.pass = 1;
@@ -1173,7 +1174,9 @@ __device__ __forceinline__ void ray_limits(Lane& L, const Ctx<ADJ>& c, const Pat
 // REFILL picks the wave's traversal discipline (see the loop body); LT
 // reads the scene's small tables from their LDS copy at `tab` (a single-job
 // launch) instead of global memory.
-template <bool ADJ, bool SSS, bool REFILL, bool LT>
+// REC: the job's path totals (Job::rec) are written (forward) or read
+// (adjoint); without it the kernel has no trace of them.
+template <bool ADJ, bool SSS, bool REFILL, bool LT, bool REC = false>
 __device__ __forceinline__ void path_loop(const PathArgs& P, int* stk, float* lds_grad, const char* tab) {
   const int stride = blockDim.x;
   const int lane = (int)__lane_id();
@@ -1212,7 +1215,8 @@ __device__ __forceinline__ void path_loop(const PathArgs& P, int* stk, float* ld
   while (true) {
     cur = __builtin_amdgcn_readfirstlane(cur);
     const Job& J = *(const Job*)(jobs + cur);
-    const Ctx<ADJ> c{J, P.adj, lds_grad, LT ? lds_tabs(J.s, J.tab, tab) : global_tabs(J.s)};
+    const Ctx<ADJ> c{J, P.adj, lds_grad, LT ? lds_tabs(J.s, J.tab, tab) : global_tabs(J.s),
+                     REC ? J.rec : nullptr};
     if (L.phase == PH_NEW) {  // a path ended: the item's next band, if any
       L.band += 1;
       L.pass = 0;
@@ -1399,13 +1403,13 @@ __device__ __forceinline__ void path_loop(const PathArgs& P, int* stk, float* ld
 // LDS of a launch: the lanes' traversal stacks (SR_STACK_SIZE entries per
 // lane, column-major), then the scene's small tables (tab.bytes, a single-job
 // launch), then the adjoint's gradient table.
-template <bool LT>
+template <bool LT, bool REC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void sr_path_kernel(PathArgs a) {
   extern __shared__ int lds_dyn[];
   char* tab = reinterpret_cast<char*>(lds_dyn + SR_STACK_SIZE * blockDim.x);
   if (threadIdx.x == 0) atomicMin(a.clocks, wall_clock64());
   if (LT) copy_tables(a.jobs[0].s, a.jobs[0].tab, tab);
-  path_loop<false, false, SR_REFILL_PATH, LT>(a, lds_dyn + threadIdx.x, nullptr, tab);
+  path_loop<false, false, SR_REFILL_PATH, LT, REC>(a, lds_dyn + threadIdx.x, nullptr, tab);
 }
 
 // The same path loop with random-walk subsurface scattering (SR_RENDER_SUBSURFACE).
@@ -1426,7 +1430,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SR_WAVES_AD
   for (int i = threadIdx.x; i < a.adj.n_params; i += blockDim.x) lds_grad[i] = 0.0f;
   if (LT) copy_tables(a.jobs[0].s, a.jobs[0].tab, tab);  // (ends with the barrier the gradient table needs)
   else __syncthreads();
-  path_loop<true, false, SR_REFILL_ADJ, LT>(a, lds_dyn + threadIdx.x, lds_grad, tab);
+  path_loop<true, false, SR_REFILL_ADJ, LT, true>(a, lds_dyn + threadIdx.x, lds_grad, tab);
   __syncthreads();
   for (int i = threadIdx.x; i < a.adj.n_params; i += blockDim.x) {
     float v = lds_grad[i];
@@ -1498,7 +1502,8 @@ static int grid_both(const void* fn_g, const void* fn_l, int block, bool adjoint
 }
 
 int sr_path_grid_bs(int block, int* grid_out) {
-  return grid_both((const void*)sr_path_kernel<false>, (const void*)sr_path_kernel<true>, block, false, grid_out);
+  return grid_both((const void*)sr_path_kernel<false, false>, (const void*)sr_path_kernel<true, false>, block, false,
+                   grid_out);
 }
 int sr_path_grid(int* grid_out) { return sr_path_grid_bs(256, grid_out); }
 int sr_adjoint_grid(int* grid_out) {
@@ -1507,8 +1512,13 @@ int sr_adjoint_grid(int* grid_out) {
 
 int sr_launch_path(const PathArgs& a, int grid, int block, hipStream_t st) {
   const size_t lds = sr_path_lds_bytes(block, false, a.tab_bytes);
-  if (a.tab_bytes) hipLaunchKernelGGL(sr_path_kernel<true>, dim3(grid), dim3(block), lds, st, a);
-  else hipLaunchKernelGGL(sr_path_kernel<false>, dim3(grid), dim3(block), lds, st, a);
+  if (a.record) {  // a single job writing its path totals
+    if (a.tab_bytes) hipLaunchKernelGGL(HIP_KERNEL_NAME(sr_path_kernel<true, true>), dim3(grid), dim3(block), lds, st, a);
+    else hipLaunchKernelGGL(HIP_KERNEL_NAME(sr_path_kernel<false, true>), dim3(grid), dim3(block), lds, st, a);
+  } else {
+    if (a.tab_bytes) hipLaunchKernelGGL(HIP_KERNEL_NAME(sr_path_kernel<true, false>), dim3(grid), dim3(block), lds, st, a);
+    else hipLaunchKernelGGL(HIP_KERNEL_NAME(sr_path_kernel<false, false>), dim3(grid), dim3(block), lds, st, a);
+  }
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
