@@ -128,16 +128,18 @@ def adjoint_ratio(reps=2):
         full.append(ds.r.last_kernel_ms())
     fwd, adj, plain, adj2, full = (min(x[1:]) for x in (fwd, adj, plain, adj2, full))
     fluoresced = torch.isnan(tot).float().mean().item()
+    untouched = ((tot == 0) & torch.signbit(tot)).float().mean().item()
     return {"value": round(adj / fwd, 3), "forward_ms": round(fwd, 2), "adjoint_ms": round(adj, 2),
             "two_pass_adjoint_ms": round(adj2, 2), "two_pass_ratio": round(adj2 / plain, 3),
             "forward_without_record_ms": round(plain, 2),
             "adjoint_over_full_forward": round(adj / full, 3), "full_forward_ms": round(full, 2),
-            "paths_replayed_twice": round(fluoresced, 4),
+            "paths_replayed_twice": round(fluoresced, 4), "paths_not_replayed": round(untouched, 4),
             "workload": f"{cfg.name}: {flat.n_tris} tris, {cfg.bands} bands, {cfg.width}x{cfg.height}, "
                         f"{cfg.spp} spp, depth {cfg.depth}, {ds.n_params} params ({segs} segments per forward; "
                         f"forward and adjoint end paths whose throughput is 0, full_forward runs every path "
                         f"to the depth limit; the forward records each path's total and the adjoint replays "
-                        f"once every path whose light stayed in its band)"}
+                        f"once every path whose light stayed in its band, and not at all a path that added "
+                        f"nothing)"}
 
 
 def bench_inverse(iterations=3, width=256, height=256, spp=64, log=print):

@@ -146,8 +146,11 @@ __device__ __forceinline__ void emit_grad(const Ctx<ADJ>& c, int idx, float v) {
 }
 
 // L.acc_band: the band L.acc is summed for, with SR_ACC_SWITCHED set once
-// a contribution of the path went to another band than the path's own.
+// a contribution of the path went to another band than the path's own and,
+// in a launch that records path totals, SR_ACC_TOUCHED once the path added
+// any (finite, nonzero) contribution.
 #define SR_ACC_SWITCHED 0x100
+#define SR_ACC_TOUCHED 0x200
 #define SR_ACC_BAND_MASK 0xFF
 
 template <bool ADJ>
@@ -184,14 +187,19 @@ __device__ __forceinline__ void add_contrib(Lane& L, const Ctx<ADJ>& c, float v)
     flush_acc<ADJ>(L, c);
     L.acc_band = L.active | SR_ACC_SWITCHED;
   }
+  if (c.rec) L.acc_band |= SR_ACC_TOUCHED;
   L.acc += v;
 }
 
 template <bool ADJ>
 __device__ __forceinline__ void end_path(Lane& L, const Ctx<ADJ>& c) {
-  // the path's total for the adjoint: L.acc holds every contribution of
-  // the path unless one went to another band
-  if (!ADJ && c.rec) c.rec[rec_index(L, c)] = (L.acc_band & SR_ACC_SWITCHED) ? NAN : L.acc;
+  // The path's total for the adjoint: L.acc holds every contribution of
+  // the path unless one went to another band (NaN).  A path that added no
+  // contribution is recorded as -0.0, which a sum of nonzero contributions
+  // started at +0.0 never is (x + -x rounds to +0.0).
+  if (!ADJ && c.rec)
+    c.rec[rec_index(L, c)] =
+        (L.acc_band & SR_ACC_SWITCHED) ? NAN : ((L.acc_band & SR_ACC_TOUCHED) ? L.acc : -0.0f);
   flush_acc<ADJ>(L, c);
   if (ADJ && L.pass == 0) {
     L.pass = 1;  // replay the same path for the gradient pass
@@ -237,6 +245,14 @@ __device__ __forceinline__ void start_path(Lane& L, const Ctx<ADJ>& c) {
         if (isfinite(tot)) {
           L.Lw = dacc(c, L, L.band) * tot;
           L.pass = 1;
+          // A path that added no contribution adds nothing to the gradient:
+          // each of its terms is (radiance still to come = 0) x (...) or the
+          // derivative of a contribution that is 0 (skipped), so it is not
+          // replayed at all.
+          if (tot == 0.0f && signbit(tot)) {
+            end_path<ADJ>(L, c);
+            return;
+          }
         }
       }
     }
