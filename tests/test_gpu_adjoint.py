@@ -132,6 +132,23 @@ def test_autograd_function(setup):
     assert np.allclose(th.grad.numpy(), g_adj[0], rtol=1e-4, atol=1e-6)
 
 
+def test_autograd_without_record_replays_twice(setup):
+    """A render whose path totals would exceed record_limit_bytes records
+    nothing, and backward runs the two-pass adjoint: the gradient of the
+    setup fixture's own two-pass call."""
+    ds, theta0, d_acc, g_adj = setup
+    limit = ds.record_limit_bytes
+    ds.record_limit_bytes = 0
+    try:
+        assert ds.new_path_totals() is None
+        th = torch.tensor(theta0, dtype=torch.float32, requires_grad=True)
+        out = render(th, ds)
+        (out * d_acc).sum().backward()
+    finally:
+        ds.record_limit_bytes = limit
+    assert np.allclose(th.grad.numpy(), g_adj[0], rtol=1e-5, atol=1e-7)
+
+
 def test_path_totals_are_the_block_images(gpu):
     """sr_render_record writes one total per path at [block][i][j][band]: on
     the box scene (no fluorescence, so every total is finite and is that
