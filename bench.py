@@ -493,7 +493,29 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration)")
+    ap.add_argument("--adjoint-once", action="store_true",
+                    help="only run one recording forward and one adjoint on the inverse-render workload "
+                         "(counter profiles of the adjoint)")
     args = ap.parse_args()
+    if args.adjoint_once:
+        import torch
+
+        from spectral import configs
+        from spectral.autograd import DifferentiableScene, inverse_volume_spec
+        from spectral.scene import flatten_scene
+
+        cfg = configs.inverse_volume()
+        flat = flatten_scene(cfg.scene, cfg.bands, cfg.wl_min, cfg.wl_max)
+        ds = DifferentiableScene(flat, inverse_volume_spec(), cfg.width, cfg.height, cfg.spp, cfg.depth, cfg.seed,
+                                 device="cuda:0")
+        d_acc = torch.rand((cfg.width, cfg.height, cfg.bands), device="cuda:0")
+        tot = ds.new_path_totals()
+        ds.render(path_totals=tot)
+        fwd = ds.r.last_kernel_ms()
+        ds.adjoint(d_acc, path_totals=tot)
+        print(json.dumps({"workload": cfg.name, "forward_ms": round(fwd, 2),
+                          "adjoint_ms": round(ds.r.last_kernel_ms(), 2)}), flush=True)
+        return
     if args.calibrate_copy:
         import torch
 
